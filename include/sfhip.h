@@ -1,0 +1,175 @@
+/* sfhip.h -- C ABI of the MI355X (gfx950) IMEX Runge-Kutta timestep.
+ *
+ * The reference has no plugin or FFI layer: its boundary is the C++ class API
+ * of IMEXRK (IMEXRK.h:31-670), StateVector (StateVector.h:8-552) and the Field
+ * family (Field.h, Stratiflow.h:8-49).  Each entry point below replaces the
+ * reference member function cited next to it; the C++14 facade in
+ * include/stratiflow/ maps those class names and signatures onto these calls.
+ *
+ * Conventions
+ *  - Every function returns an int status: SF_OK (0) or an SF_ERR_* code; the
+ *    message is available from sf_last_error(ctx).  No C++ exception crosses
+ *    this boundary.
+ *  - Host arrays use the reference's layouts (Field.h:399-410) in double:
+ *      modal : complex, interleaved re/im, index ((N1/2+1)*j2 + j1)*N3 + j3
+ *      nodal : real,    index (N1*n2 + n1)*N3 + n3
+ *      1D    : real,    index j3
+ *    Device storage of modal fields keeps only the active (2/3-dealiased)
+ *    modes (ModalField::ParallelPerStack, Field.h:1129-1166); uploading a
+ *    modal array therefore applies ModalField::Filter (Field.h:996-1021).
+ *  - One context owns one HIP stream; calls are stream-ordered and
+ *    asynchronous except downloads, reductions and sf_synchronize.
+ */
+#ifndef SFHIP_H
+#define SFHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SF_OK 0
+#define SF_ERR_ARG 1
+#define SF_ERR_HIP 2
+#define SF_ERR_UNSUPPORTED 3
+#define SF_ERR_ALLOC 4
+#define SF_ERR_STATE 5
+
+/* Dimensionality, Constants.h:84-89 */
+#define SF_THREE_D 0
+#define SF_TWO_D 1
+#define SF_TWO_AND_A_HALF 2
+
+/* BoundaryCondition, Constants.h:78-82 */
+#define SF_NEUMANN 0
+#define SF_DIRICHLET 1
+
+/* field kinds: NodalField, ModalField, Nodal1D (Field.h:541-1207) */
+#define SF_NODAL 0
+#define SF_MODAL 1
+#define SF_1D 2
+
+/* solver-owned fields for sf_state_field (IMEXRK.h:605-619, 640-641) */
+#define SF_U1 0
+#define SF_U2 1
+#define SF_U3 2
+#define SF_B 3
+#define SF_P 4
+#define SF_NODAL_U1 5
+#define SF_NODAL_U2 6
+#define SF_NODAL_U3 7
+#define SF_NODAL_B 8
+#define SF_R1 9
+#define SF_R2 10
+#define SF_R3 11
+#define SF_RB 12
+#define SF_U1_TOT 13 /* linearisation state u1_tot .. b_tot (IMEXRK.h:614-615) */
+#define SF_U2_TOT 14
+#define SF_U3_TOT 15
+#define SF_B_TOT 16
+
+/* GridParams, Parameters.h:6-18 (runtime here, constexpr in the reference) */
+typedef struct sf_grid {
+    int N1, N2, N3;
+    int dimensionality;
+} sf_grid;
+
+/* FlowParams, Parameters.h:21-30 */
+typedef struct sf_flow {
+    double L1, L2, L3, Re, Ri, Pr;
+    int evolve_background;
+} sf_flow;
+
+typedef struct sf_ctx sf_ctx;
+typedef struct sf_field sf_field;
+
+/* ---- context --------------------------------------------------------- */
+/* IMEXRK::IMEXRK (IMEXRK.h:37-81).  precision: 4 (float, the reference's
+ * default stratifloat) or 8 (double, -DDOUBLE=On). */
+int sf_ctx_create(const sf_grid* grid, const sf_flow* flow, int precision, sf_ctx** out);
+int sf_ctx_destroy(sf_ctx* ctx);
+/* message of the last failed call on ctx (or of the last failed
+ * sf_ctx_create when ctx is NULL) */
+const char* sf_last_error(const sf_ctx* ctx);
+/* IMEXRK::deltaT + UpdateForTimestep (IMEXRK.h:34, 502-545) */
+int sf_set_dt(sf_ctx* ctx, double dt);
+int sf_get_dt(const sf_ctx* ctx, double* dt);
+/* StateVector::ResetForParams (StateVector.h:505-508) after flowParams changes */
+int sf_set_flow(sf_ctx* ctx, const sf_flow* flow);
+/* IMEXRK::SetBackground(const Neumann1D&) (IMEXRK.h:321-324): U is the N3
+ * values of U_.  dBdz (N3 values at Neumann points, may be NULL = 1) is the
+ * background stratification gradient that IMEXRK.cpp:145-147 fixes at 1. */
+int sf_set_background(sf_ctx* ctx, const double* U, const double* dBdz);
+/* VerticalPoints and VerticalPointsFractional (Field.cpp:3-31): N3 values
+ * each (z[0] is extrapolated; the reference leaves it uninitialised) */
+int sf_vertical_points(const sf_ctx* ctx, double* z, double* zf);
+
+/* ---- fields ---------------------------------------------------------- */
+/* NeumannNodal/DirichletNodal/NeumannModal/DirichletModal/Neumann1D
+ * constructors (Stratiflow.h:8-49) */
+int sf_field_alloc(sf_ctx* ctx, int kind, int bc, sf_field** out);
+int sf_field_free(sf_field* f);
+int sf_field_kind(const sf_field* f, int* kind, int* bc);
+/* host <-> device in the reference layout (Field::Raw, Field.h:418-431) */
+int sf_field_upload(sf_field* f, const double* host);
+int sf_field_download(const sf_field* f, double* host);
+/* Field::operator= (Field.h:285-305) */
+int sf_field_copy(sf_field* dst, const sf_field* src);
+/* Field::Zero (Field.h:433-440) */
+int sf_field_zero(sf_field* f);
+/* Field::operator*= (Field.h:342-350) */
+int sf_field_scale(sf_field* f, double a);
+/* y = a*x + b*y; covers Field::operator+=, -= and StateVector::MulAdd
+ * (Field.h:352-377, StateVector.h:67-78) */
+int sf_field_axpby(sf_field* y, double a, const sf_field* x, double b);
+/* ZeroEnds (Field.h:475-486), NeumannEnds (Field.h:488-500) */
+int sf_field_zero_ends(sf_field* f);
+int sf_field_neumann_ends(sf_field* f);
+/* ModalField::Filter (Field.h:996-1021): a no-op on compact storage */
+int sf_field_filter(sf_field* f);
+/* NodalField::ToModal (Field.h:734-769), ModalField::ToNodal (Field.h:965-994) */
+int sf_to_modal(const sf_field* nodal, sf_field* modal);
+int sf_to_nodal(const sf_field* modal, sf_field* nodal);
+/* ddx, ddy, ddz (Stratiflow.h:51-80); ddz maps Neumann->Dirichlet and
+ * Dirichlet->Neumann; out must have the resulting boundary condition */
+int sf_ddx(const sf_field* in, sf_field* out);
+int sf_ddy(const sf_field* in, sf_field* out);
+int sf_ddz(const sf_field* in, sf_field* out);
+/* InnerProd (Integration.h:117-146) accumulated in FP64 on the GPU */
+int sf_inner_prod(const sf_field* a, const sf_field* b, double* out);
+
+/* ---- stepper (the IMEXRK solver state) ------------------------------- */
+/* borrow one of the solver's own fields (SF_U1 .. SF_B_TOT); the handle is
+ * owned by the context and must not be freed */
+int sf_state_field(sf_ctx* ctx, int which, sf_field** out);
+/* IMEXRK::TimeStep (IMEXRK.cpp:8-24), nsteps times */
+int sf_step(sf_ctx* ctx, int nsteps);
+/* IMEXRK::TimeStepLinear (IMEXRK.cpp:26-41) about the state set with
+ * sf_set_linear_background */
+int sf_step_linear(sf_ctx* ctx, int nsteps);
+/* IMEXRK::SetBackground(u1,u2,u3,b) (IMEXRK.h:334-352): copies SF_U1_TOT..
+ * SF_B_TOT from the given modal fields (u2 may be NULL in 2D) */
+int sf_set_linear_background(sf_ctx* ctx, const sf_field* u1, const sf_field* u2, const sf_field* u3,
+                             const sf_field* b);
+/* one RK stage k of TimeStep (0..2) or TimeStepLinear (linear != 0) */
+int sf_stage(sf_ctx* ctx, int k, int linear);
+/* IMEXRK::RemoveDivergence (IMEXRK.cpp:43-73) */
+int sf_remove_divergence(sf_ctx* ctx, double pressure_multiplier);
+/* IMEXRK::FilterAll (IMEXRK.h:128-139) */
+int sf_filter_all(sf_ctx* ctx);
+/* IMEXRK::PopulateNodalVariables (IMEXRK.h:141-162) */
+int sf_populate_nodal(sf_ctx* ctx);
+/* IMEXRK::KE and PE (IMEXRK.h:395-411), FP64 reductions */
+int sf_energy(sf_ctx* ctx, double* ke, double* pe);
+/* IMEXRK::CFL (IMEXRK.h:355-374) without the dt update: returns the CFL
+ * number for the current dt from the nodal fields (populates them) */
+int sf_cfl(sf_ctx* ctx, double* cfl);
+int sf_synchronize(sf_ctx* ctx);
+/* nsteps of sf_step bracketed by HIP events on the context's stream;
+ * *ms = elapsed milliseconds */
+int sf_time_steps(sf_ctx* ctx, int nsteps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFHIP_H */

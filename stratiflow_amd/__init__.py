@@ -1,0 +1,341 @@
+"""Host-side mirror of Stratiflow's IMEXRK / Field interface over ``libsfhip.so``.
+
+The product is the gfx950 library ``libsfhip.so`` with the C ABI declared in
+``include/sfhip.h``.  This module binds that ABI with ctypes so that Python
+callers (the test-suite, ``bench.py``) drive the same entry points a C++
+caller uses through ``include/stratiflow/``.  There is no fallback: if the
+library is missing or no GPU is present, every call raises.
+
+Names follow the reference: ``IMEXRK`` (IMEXRK.h:31-670) with ``TimeStep``,
+``RemoveDivergence``, ``FilterAll``, ``PopulateNodalVariables``, ``KE``,
+``PE``, ``SetBackground``, ``deltaT``; fields follow ``NeumannModal`` /
+``DirichletModal`` / ``NeumannNodal`` / ``DirichletNodal`` (Stratiflow.h:8-49).
+Host arrays use the reference layouts (Field.h:399-410): modal fields are
+complex ``(N2, N1//2+1, N3)``, nodal fields real ``(N2, N1, N3)``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsfhip.so")
+
+THREE_D, TWO_D, TWO_AND_A_HALF = 0, 1, 2
+NEUMANN, DIRICHLET = 0, 1
+NODAL, MODAL, ONE_D = 0, 1, 2
+STATE_IDS = {"u1": 0, "u2": 1, "u3": 2, "b": 3, "p": 4, "U1": 5, "U2": 6, "U3": 7, "B": 8,
+             "r1": 9, "r2": 10, "r3": 11, "rB": 12, "u1_tot": 13, "u2_tot": 14, "u3_tot": 15, "b_tot": 16}
+
+
+class SFError(RuntimeError):
+    """Non-zero status from the C ABI (message from sf_last_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__("sfhip error %d: %s" % (code, msg))
+        self.code = code
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [("N1", ctypes.c_int), ("N2", ctypes.c_int), ("N3", ctypes.c_int), ("dimensionality", ctypes.c_int)]
+
+
+class _Flow(ctypes.Structure):
+    _fields_ = [("L1", ctypes.c_double), ("L2", ctypes.c_double), ("L3", ctypes.c_double),
+                ("Re", ctypes.c_double), ("Ri", ctypes.c_double), ("Pr", ctypes.c_double),
+                ("evolve_background", ctypes.c_int)]
+
+
+_lib = None
+
+# every extern "C" symbol of include/sfhip.h
+EXPORTED = [
+    "sf_ctx_create", "sf_ctx_destroy", "sf_last_error", "sf_set_dt", "sf_get_dt", "sf_set_flow",
+    "sf_set_background", "sf_vertical_points", "sf_field_alloc", "sf_field_free", "sf_field_kind", "sf_field_upload",
+    "sf_field_download", "sf_field_copy", "sf_field_zero", "sf_field_scale", "sf_field_axpby",
+    "sf_field_zero_ends", "sf_field_neumann_ends", "sf_field_filter", "sf_to_modal", "sf_to_nodal",
+    "sf_ddx", "sf_ddy", "sf_ddz", "sf_inner_prod", "sf_state_field", "sf_step", "sf_step_linear",
+    "sf_set_linear_background", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
+    "sf_energy", "sf_cfl", "sf_synchronize", "sf_time_steps",
+]
+
+
+def lib():
+    """Load libsfhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libsfhip.so not found at %s: run `python -m stratiflow_amd.build`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    dp = ctypes.POINTER(ctypes.c_double)
+    vpp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "sf_ctx_create": [ctypes.POINTER(_Grid), ctypes.POINTER(_Flow), i, vpp],
+        "sf_ctx_destroy": [vp],
+        "sf_set_dt": [vp, d],
+        "sf_get_dt": [vp, dp],
+        "sf_set_flow": [vp, ctypes.POINTER(_Flow)],
+        "sf_set_background": [vp, dp, dp],
+        "sf_vertical_points": [vp, dp, dp],
+        "sf_field_alloc": [vp, i, i, vpp],
+        "sf_field_free": [vp],
+        "sf_field_kind": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+        "sf_field_upload": [vp, dp],
+        "sf_field_download": [vp, dp],
+        "sf_field_copy": [vp, vp],
+        "sf_field_zero": [vp],
+        "sf_field_scale": [vp, d],
+        "sf_field_axpby": [vp, d, vp, d],
+        "sf_field_zero_ends": [vp],
+        "sf_field_neumann_ends": [vp],
+        "sf_field_filter": [vp],
+        "sf_to_modal": [vp, vp],
+        "sf_to_nodal": [vp, vp],
+        "sf_ddx": [vp, vp],
+        "sf_ddy": [vp, vp],
+        "sf_ddz": [vp, vp],
+        "sf_inner_prod": [vp, vp, dp],
+        "sf_state_field": [vp, i, vpp],
+        "sf_step": [vp, i],
+        "sf_step_linear": [vp, i],
+        "sf_set_linear_background": [vp, vp, vp, vp, vp],
+        "sf_stage": [vp, i, i],
+        "sf_remove_divergence": [vp, d],
+        "sf_filter_all": [vp],
+        "sf_populate_nodal": [vp],
+        "sf_energy": [vp, dp, dp],
+        "sf_cfl": [vp, dp],
+        "sf_synchronize": [vp],
+        "sf_time_steps": [vp, i, dp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L.sf_last_error.argtypes = [vp]
+    L.sf_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Field:
+    """A device field (NeumannModal, DirichletNodal, ...), owned or borrowed."""
+
+    def __init__(self, solver, handle, kind, bc, owned):
+        self.solver = solver
+        self.h = handle
+        self.kind = kind
+        self.bc = bc
+        self.owned = owned
+
+    def __del__(self):
+        if self.owned and getattr(self, "h", None):
+            try:
+                lib().sf_field_free(self.h)
+            except Exception:
+                pass
+            self.h = None
+
+    @property
+    def shape(self):
+        s = self.solver
+        if self.kind == MODAL:
+            return (s.N2, s.M1, s.N3)
+        if self.kind == NODAL:
+            return (s.N2, s.N1, s.N3)
+        return (s.N3,)
+
+    def upload(self, arr):
+        if self.kind == MODAL:
+            a = np.ascontiguousarray(np.asarray(arr, dtype=np.complex128).reshape(self.shape))
+            self.solver._chk(lib().sf_field_upload(self.h, _p(a.view(np.float64))))
+        else:
+            a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).reshape(self.shape))
+            self.solver._chk(lib().sf_field_upload(self.h, _p(a)))
+
+    def download(self):
+        if self.kind == MODAL:
+            a = np.zeros(self.shape, dtype=np.complex128)
+            self.solver._chk(lib().sf_field_download(self.h, _p(a.view(np.float64))))
+        else:
+            a = np.zeros(self.shape, dtype=np.float64)
+            self.solver._chk(lib().sf_field_download(self.h, _p(a)))
+        return a
+
+    # Field / ModalField / NodalField operations
+    def Zero(self):
+        self.solver._chk(lib().sf_field_zero(self.h))
+
+    def assign(self, other):
+        self.solver._chk(lib().sf_field_copy(self.h, other.h))
+
+    def __imul__(self, a):
+        self.solver._chk(lib().sf_field_scale(self.h, float(a)))
+        return self
+
+    def axpby(self, a, x, b):
+        self.solver._chk(lib().sf_field_axpby(self.h, float(a), x.h if x is not None else None, float(b)))
+
+    def MulAdd(self, a, x):
+        self.axpby(a, x, 1.0)
+
+    def ZeroEnds(self):
+        self.solver._chk(lib().sf_field_zero_ends(self.h))
+
+    def NeumannEnds(self):
+        self.solver._chk(lib().sf_field_neumann_ends(self.h))
+
+    def Filter(self):
+        self.solver._chk(lib().sf_field_filter(self.h))
+
+    def ToModal(self, other):
+        self.solver._chk(lib().sf_to_modal(self.h, other.h))
+
+    def ToNodal(self, other):
+        self.solver._chk(lib().sf_to_nodal(self.h, other.h))
+
+
+class IMEXRK:
+    """The IMEX-RK3/CN solver of IMEXRK.h:31-670 on one GPU.
+
+    ``precision`` 4 is the reference's default ``stratifloat`` (float), 8 the
+    ``-DDOUBLE=On`` build.
+    """
+
+    def __init__(self, N1, N2, N3, dimensionality, L1, L2, L3, Re, Ri, Pr, precision=4):
+        L = lib()
+        self.N1, self.N2, self.N3 = N1, N2, N3
+        self.M1 = N1 // 2 + 1
+        self.dimensionality = dimensionality
+        self.precision = precision
+        self.flow = dict(L1=L1, L2=L2, L3=L3, Re=Re, Ri=Ri, Pr=Pr)
+        g = _Grid(N1, N2, N3, dimensionality)
+        f = _Flow(L1, L2, L3, Re, Ri, Pr, 0)
+        h = ctypes.c_void_p()
+        rc = L.sf_ctx_create(ctypes.byref(g), ctypes.byref(f), precision, ctypes.byref(h))
+        if rc != 0:
+            raise SFError(rc, L.sf_last_error(None).decode())
+        self.h = h
+        self._state = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._state = {}
+            lib().sf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SFError(rc, lib().sf_last_error(self.h).decode())
+
+    # ---- parameters ----
+    @property
+    def deltaT(self):
+        d = ctypes.c_double()
+        self._chk(lib().sf_get_dt(self.h, ctypes.byref(d)))
+        return d.value
+
+    @deltaT.setter
+    def deltaT(self, dt):
+        """IMEXRK::deltaT followed by UpdateForTimestep (IMEXRK.h:502-545)."""
+        self._chk(lib().sf_set_dt(self.h, float(dt)))
+
+    def UpdateForTimestep(self):
+        self.deltaT = self.deltaT
+
+    def ResetForParams(self, **flow):
+        self.flow.update(flow)
+        f = _Flow(self.flow["L1"], self.flow["L2"], self.flow["L3"], self.flow["Re"], self.flow["Ri"],
+                  self.flow["Pr"], 0)
+        self._chk(lib().sf_set_flow(self.h, ctypes.byref(f)))
+
+    def vertical_points(self):
+        """(z, zf): VerticalPoints / VerticalPointsFractional (Field.cpp:3-31)."""
+        z, zf = np.zeros(self.N3), np.zeros(self.N3)
+        self._chk(lib().sf_vertical_points(self.h, _p(z), _p(zf)))
+        return z, zf
+
+    def SetBackground(self, U, dBdz=None):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        S = None if dBdz is None else np.ascontiguousarray(dBdz, dtype=np.float64)
+        self._chk(lib().sf_set_background(self.h, _p(U), _p(S) if S is not None else None))
+
+    # ---- fields ----
+    def field(self, kind, bc):
+        h = ctypes.c_void_p()
+        self._chk(lib().sf_field_alloc(self.h, kind, bc, ctypes.byref(h)))
+        return Field(self, h, kind, bc, True)
+
+    def state(self, name):
+        if name not in self._state:
+            h = ctypes.c_void_p()
+            self._chk(lib().sf_state_field(self.h, STATE_IDS[name], ctypes.byref(h)))
+            k, b = ctypes.c_int(), ctypes.c_int()
+            self._chk(lib().sf_field_kind(h, ctypes.byref(k), ctypes.byref(b)))
+            self._state[name] = Field(self, h, k.value, b.value, False)
+        return self._state[name]
+
+    def get(self, name):
+        return self.state(name).download()
+
+    def set(self, name, arr):
+        self.state(name).upload(arr)
+
+    # ---- stepping ----
+    def TimeStep(self, n=1):
+        self._chk(lib().sf_step(self.h, int(n)))
+
+    def TimeStepLinear(self, n=1):
+        self._chk(lib().sf_step_linear(self.h, int(n)))
+
+    def Stage(self, k, linear=False):
+        self._chk(lib().sf_stage(self.h, int(k), int(bool(linear))))
+
+    def SetLinearBackground(self, u1, u2, u3, b):
+        """IMEXRK::SetBackground(u1, u2, u3, b) with modal Field objects."""
+        self._chk(lib().sf_set_linear_background(self.h, u1.h, u2.h if u2 is not None else None, u3.h, b.h))
+
+    def RemoveDivergence(self, pressureMultiplier=1.0):
+        self._chk(lib().sf_remove_divergence(self.h, float(pressureMultiplier)))
+
+    def FilterAll(self):
+        self._chk(lib().sf_filter_all(self.h))
+
+    def PopulateNodalVariables(self):
+        self._chk(lib().sf_populate_nodal(self.h))
+
+    def energies(self):
+        ke, pe = ctypes.c_double(), ctypes.c_double()
+        self._chk(lib().sf_energy(self.h, ctypes.byref(ke), ctypes.byref(pe)))
+        return ke.value, pe.value
+
+    def KE(self):
+        return self.energies()[0]
+
+    def PE(self):
+        return self.energies()[1]
+
+    def InnerProd(self, a, b):
+        out = ctypes.c_double()
+        self._chk(lib().sf_inner_prod(a.h, b.h, ctypes.byref(out)))
+        return out.value
+
+    def synchronize(self):
+        self._chk(lib().sf_synchronize(self.h))
+
+    def time_steps(self, n):
+        ms = ctypes.c_double()
+        self._chk(lib().sf_time_steps(self.h, int(n), ctypes.byref(ms)))
+        return ms.value

@@ -1,0 +1,1031 @@
+// sf_api.hip -- context, fields and the extern "C" boundary (include/sfhip.h).
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sfhip.h"
+#include "sf_kernels.h"
+
+namespace sf {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SF_HIP(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) throw ::sf::Error(SF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define SF_REQUIRE(cond, code, msg)                  \
+    do {                                             \
+        if (!(cond)) throw ::sf::Error(code, msg);   \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Vertical grid (Field.cpp:3-31, 62-83) in double.  z[0] and DY[0], which
+// the reference leaves uninitialised, get extrapolated values.
+// ---------------------------------------------------------------------------
+struct VGrid {
+    std::vector<double> z, zf, DY, DYF;
+};
+
+static VGrid vertical_grid(double L, int N) {
+    VGrid g;
+    g.z.assign(N, 0.0);
+    g.zf.assign(N, 0.0);
+    g.DY.assign(N, 0.0);
+    g.DYF.assign(N, 0.0);
+    const int M = N - 1;  // points 1..N-1 from LinSpaced(N-1, -1, 1)
+    const double step = 2.0 / double(M - 1);
+    for (int t = 0; t < M; ++t) {
+        const double x = (t < M / 2) ? -1.0 + t * step : 1.0 - (M - 1 - t) * step;
+        g.z[1 + t] = L * (2.0 * std::pow(x, 7) + x) / 3.0;
+    }
+    const double scale = -L / (0.5 * (g.z[1] + g.z[2]));
+    for (int t = 1; t < N; ++t) g.z[t] *= scale;
+    g.z[0] = 2.0 * g.z[1] - g.z[2];
+    for (int j = 1; j <= N - 2; ++j) g.zf[j] = 0.5 * (g.z[j] + g.z[j + 1]);
+    g.zf[0] = 2.0 * g.zf[1] - g.zf[2];
+    g.zf[N - 1] = 2.0 * g.zf[N - 2] - g.zf[N - 3];
+    for (int j = 1; j < N; ++j) g.DY[j] = g.zf[j] - g.zf[j - 1];
+    g.DY[0] = g.DY[1];
+    for (int j = 1; j <= N - 2; ++j) g.DYF[j] = g.z[j + 1] - g.z[j];
+    g.DYF[N - 1] = g.DYF[N - 2];
+    g.DYF[0] = g.DYF[1];
+    return g;
+}
+
+// integer wavenumber of Fourier index j (Differentiation.cpp:113-132)
+static int kindex(int n, int j) {
+    if (n == 1) return 0;
+    if (j <= n / 3) return j;
+    if (j < n - n / 3) return 0;
+    return j - n;
+}
+
+struct FieldBase {
+    virtual ~FieldBase() {}
+};
+
+}  // namespace sf
+
+struct sf_field {
+    sf_ctx* ctx;
+    int kind;
+    int bc;
+    size_t n;       // scalars of type T (complex counts twice)
+    void* d;        // device pointer
+    bool owned;     // false for solver-owned fields
+};
+
+struct sf_ctx {
+    int precision;
+    std::string err;
+    std::shared_ptr<void> impl;  // sf::Ctx<T>
+    void* raw;
+};
+
+namespace sf {
+
+static thread_local std::string g_create_error;
+
+template <typename T>
+struct Ctx {
+    using V = cx<T>;
+    sf_ctx* owner = nullptr;
+    int N1 = 0, N2 = 0, N3 = 0, M1 = 0, A1 = 0, A2 = 0, dim = 0, N3p = 0;
+    bool three = false;
+    double L1 = 0, L2 = 0, L3 = 0, Re = 0, Ri = 0, Pr = 0;
+    double dt = 0.01;
+    T h[3] = {0, 0, 0};
+    size_t nmodal = 0;  // complex values per modal field
+    size_t nnodal = 0;  // real values per nodal field
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+
+    T* zc = nullptr;
+    T *kx = nullptr, *ky = nullptr, *kx2 = nullptr, *ky2 = nullptr;
+    V *tw1 = nullptr, *tw2 = nullptr;
+    double* partial = nullptr;
+    double* scal = nullptr;
+    double* scal_host = nullptr;
+    std::vector<double> zc_host;  // double copy of the coefficient table
+
+    sf_field state[17];
+    V* rbuf[2][4] = {{nullptr}};
+    int rcur = 0;
+    V* nl[4] = {nullptr};
+    V* Y[4] = {nullptr};
+    V* Yb[4] = {nullptr};
+    V* X[8] = {nullptr};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    template <typename P>
+    P* dalloc(size_t count) {
+        void* p = nullptr;
+        const size_t bytes = count * sizeof(P);
+        SF_HIP(hipMalloc(&p, bytes > 0 ? bytes : 16));
+        allocs.push_back(p);
+        SF_HIP(hipMemsetAsync(p, 0, bytes > 0 ? bytes : 16, stream));
+        return static_cast<P*>(p);
+    }
+
+    ~Ctx() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : allocs) (void)hipFree(p);
+        if (scal_host) (void)hipHostFree(scal_host);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void init(const sf_grid& g, const sf_flow& f) {
+        N1 = g.N1;
+        N2 = g.N2;
+        N3 = g.N3;
+        dim = g.dimensionality;
+        SF_REQUIRE(dim == SF_THREE_D || dim == SF_TWO_D, SF_ERR_UNSUPPORTED,
+                   "dimensionality must be SF_THREE_D or SF_TWO_D");
+        three = (dim == SF_THREE_D);
+        SF_REQUIRE(N3 >= 8 && N3 % 4 == 0 && N3 <= 64 * max_rows_per_lane(), SF_ERR_ARG,
+                   "N3 must be a multiple of 4 in [8, 512] (Field.cpp:7)");
+        SF_REQUIRE(tz_supported(N1), SF_ERR_UNSUPPORTED, "unsupported N1 (FFT sizes: 6..512 of the form 2^a*3^b)");
+        if (three) {
+            SF_REQUIRE(N2 >= 6 && tz_supported(N2), SF_ERR_UNSUPPORTED, "unsupported N2 for 3D");
+        } else {
+            SF_REQUIRE(N2 == 1, SF_ERR_ARG, "2D requires N2 == 1 (IMEXRK.h:43)");
+        }
+        SF_REQUIRE(!f.evolve_background, SF_ERR_UNSUPPORTED, "EvolveBackground is not supported");
+        M1 = N1 / 2 + 1;
+        A1 = N1 / 3;
+        A2 = three ? 2 * (N2 / 3) - 1 : 1;
+        N3p = N3;
+        nmodal = size_t(A1) * A2 * N3;
+        nnodal = size_t(N1) * N2 * N3;
+        SF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        SF_HIP(hipEventCreate(&ev0));
+        SF_HIP(hipEventCreate(&ev1));
+        SF_HIP(hipHostMalloc((void**)&scal_host, 16 * sizeof(double), hipHostMallocDefault));
+
+        zc = dalloc<T>(size_t(ZC_COUNT) * N3p);
+        kx = dalloc<T>(A1);
+        kx2 = dalloc<T>(A1);
+        ky = dalloc<T>(A2);
+        ky2 = dalloc<T>(A2);
+        tw1 = dalloc<V>(N1);
+        tw2 = dalloc<V>(N2);
+        partial = dalloc<double>(2048);
+        scal = dalloc<double>(16);
+
+        // solver fields
+        const int mk[17][2] = {{SF_MODAL, SF_NEUMANN},   {SF_MODAL, SF_NEUMANN},   {SF_MODAL, SF_DIRICHLET},
+                               {SF_MODAL, SF_NEUMANN},   {SF_MODAL, SF_NEUMANN},   {SF_NODAL, SF_NEUMANN},
+                               {SF_NODAL, SF_NEUMANN},   {SF_NODAL, SF_DIRICHLET}, {SF_NODAL, SF_NEUMANN},
+                               {SF_MODAL, SF_NEUMANN},   {SF_MODAL, SF_NEUMANN},   {SF_MODAL, SF_DIRICHLET},
+                               {SF_MODAL, SF_NEUMANN},   {SF_MODAL, SF_NEUMANN},   {SF_MODAL, SF_NEUMANN},
+                               {SF_MODAL, SF_DIRICHLET}, {SF_MODAL, SF_NEUMANN}};
+        for (int i = 0; i < 17; ++i) {
+            sf_field& s = state[i];
+            s.ctx = owner;
+            s.kind = mk[i][0];
+            s.bc = mk[i][1];
+            s.owned = false;
+            s.n = s.kind == SF_MODAL ? 2 * nmodal : nnodal;
+            if (i >= SF_R1 && i <= SF_RB) {
+                s.d = nullptr;
+                continue;
+            }
+            s.d = (s.kind == SF_MODAL) ? (void*)dalloc<V>(nmodal) : (void*)dalloc<T>(nnodal);
+        }
+        for (int k = 0; k < 2; ++k)
+            for (int i = 0; i < 4; ++i) rbuf[k][i] = dalloc<V>(nmodal);
+        rcur = 0;
+        bind_r();
+        for (int i = 0; i < 4; ++i) nl[i] = dalloc<V>(nmodal);
+        if (three) {
+            const size_t ny = size_t(N2) * A1 * N3;
+            for (int i = 0; i < 4; ++i) Y[i] = dalloc<V>(ny);
+            for (int i = 0; i < 8; ++i) X[i] = dalloc<V>(ny);
+        }
+        set_flow(f);
+        std::vector<double> U(N3, 0.0);
+        set_background(U.data(), nullptr);
+        set_dt(0.01);
+        SF_HIP(hipStreamSynchronize(stream));
+    }
+
+    void bind_r() {
+        for (int i = 0; i < 4; ++i) state[SF_R1 + i].d = rbuf[rcur][i];
+    }
+
+    void upload_coefficients() {
+        std::vector<T> h(zc_host.size());
+        for (size_t i = 0; i < h.size(); ++i) h[i] = T(zc_host[i]);
+        SF_HIP(hipMemcpyAsync(zc, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+    }
+
+    void set_flow(const sf_flow& f) {
+        SF_REQUIRE(f.L1 > 0 && f.L2 > 0 && f.L3 > 0 && f.Re > 0 && f.Pr > 0, SF_ERR_ARG, "bad flow parameters");
+        L1 = f.L1;
+        L2 = f.L2;
+        L3 = f.L3;
+        Re = f.Re;
+        Ri = f.Ri;
+        Pr = f.Pr;
+        // vertical operators (Differentiation.cpp:7-111), banded
+        const VGrid g = vertical_grid(L3, N3);
+        std::vector<double> old = zc_host;
+        zc_host.assign(size_t(ZC_COUNT) * N3p, 0.0);
+        auto Z = [&](int c, int j) -> double& { return zc_host[size_t(c) * N3p + j]; };
+        const int N = N3;
+        for (int j = 1; j < N - 1; ++j) {
+            const double lo = 1.0 / g.DY[j] / g.DYF[j], up = 1.0 / g.DY[j + 1] / g.DYF[j];
+            Z(ZC_DZZN_LO, j) = lo;
+            Z(ZC_DZZN_UP, j) = up;
+            Z(ZC_DZZN_DI, j) = -lo - up;
+        }
+        for (int j = 2; j < N - 1; ++j) {
+            const double lo = 1.0 / g.DYF[j - 1] / g.DY[j], up = 1.0 / g.DYF[j] / g.DY[j];
+            Z(ZC_DZZD_LO, j) = lo;
+            Z(ZC_DZZD_UP, j) = up;
+            Z(ZC_DZZD_DI, j) = -lo - up;
+        }
+        for (int j = 2; j <= N - 1; ++j) Z(ZC_DZND, j) = 1.0 / g.DY[j];
+        for (int j = 1; j <= N - 1; ++j) Z(ZC_DZDN, j) = 1.0 / g.DYF[j];
+        for (int j = 1; j < N; ++j) {
+            Z(ZC_IFULL_LO, j) = g.DYF[j] / (2.0 * g.DY[j]);
+            Z(ZC_IFULL_DI, j) = g.DYF[j - 1] / (2.0 * g.DY[j]);
+            Z(ZC_ITIL_LO, j) = g.DYF[j - 1] / (2.0 * g.DY[j]);
+            Z(ZC_ITIL_DI, j) = g.DYF[j] / (2.0 * g.DY[j]);
+        }
+        // InnerProd weights, IntegrateVertically (Integration.h:5-35)
+        for (int j = 1; j < N - 1; ++j) Z(ZC_WN, j) = g.z[j + 1] - g.z[j];
+        for (int j = 2; j < N - 1; ++j) Z(ZC_WD, j) = g.zf[j] - g.zf[j - 1];
+        // keep the background profiles
+        for (int j = 0; j < N; ++j) {
+            Z(ZC_UBAR, j) = old.empty() ? 0.0 : old[size_t(ZC_UBAR) * N3p + j];
+            Z(ZC_S, j) = old.empty() ? 1.0 : old[size_t(ZC_S) * N3p + j];
+        }
+        upload_coefficients();
+        // Fourier wavenumbers of the active modes (Differentiation.cpp:113-160)
+        std::vector<T> hkx(A1), hkx2(A1), hky(A2), hky2(A2);
+        for (int j = 0; j < A1; ++j) {
+            const double k = 2.0 * M_PI * kindex(N1, j) / L1;
+            hkx[j] = T(k);
+            hkx2[j] = T(-k * k);
+        }
+        for (int ja = 0; ja < A2; ++ja) {
+            const int j2 = three ? ((ja < N2 / 3) ? ja : ja + N2 - 2 * (N2 / 3) + 1) : 0;
+            const double k = 2.0 * M_PI * kindex(N2, j2) / L2;
+            hky[ja] = T(k);
+            hky2[ja] = T(-k * k);
+        }
+        SF_HIP(hipMemcpyAsync(kx, hkx.data(), A1 * sizeof(T), hipMemcpyHostToDevice, stream));
+        SF_HIP(hipMemcpyAsync(kx2, hkx2.data(), A1 * sizeof(T), hipMemcpyHostToDevice, stream));
+        SF_HIP(hipMemcpyAsync(ky, hky.data(), A2 * sizeof(T), hipMemcpyHostToDevice, stream));
+        SF_HIP(hipMemcpyAsync(ky2, hky2.data(), A2 * sizeof(T), hipMemcpyHostToDevice, stream));
+        // twiddles exp(-2 pi i k/N), computed in double
+        auto tw = [&](int n, V* dst) {
+            std::vector<V> w(n);
+            for (int k = 0; k < n; ++k) {
+                const double a = -2.0 * M_PI * double(k) / double(n);
+                w[k].x = T(std::cos(a));
+                w[k].y = T(std::sin(a));
+            }
+            SF_HIP(hipMemcpyAsync(dst, w.data(), n * sizeof(V), hipMemcpyHostToDevice, stream));
+            SF_HIP(hipStreamSynchronize(stream));
+        };
+        tw(N1, tw1);
+        tw(N2, tw2);
+        SF_HIP(hipStreamSynchronize(stream));
+    }
+
+    void set_background(const double* U, const double* S) {
+        for (int j = 0; j < N3; ++j) {
+            zc_host[size_t(ZC_UBAR) * N3p + j] = U ? U[j] : 0.0;
+            zc_host[size_t(ZC_S) * N3p + j] = S ? S[j] : 1.0;
+        }
+        upload_coefficients();
+    }
+
+    // UpdateForTimestep (IMEXRK.h:506-508); the implicit matrices are rebuilt
+    // inside k_modal, so nothing else is refactorised.
+    void set_dt(double d) {
+        SF_REQUIRE(d > 0 && std::isfinite(d), SF_ERR_ARG, "dt must be positive");
+        dt = d;
+        h[0] = T(d * 8.0 / 15.0);
+        h[1] = T(d * 2.0 / 15.0);
+        h[2] = T(d * 5.0 / 15.0);
+    }
+
+    V* mf(int i) { return static_cast<V*>(state[i].d); }
+
+    void check(hipError_t e, const char* what) {
+        if (e != hipSuccess) throw Error(SF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+
+    // nonlinear (or linearised) explicit terms of the current state into nl[]
+    void transform_path(bool linear) {
+        if (three) {
+            YArgs<T> a{};
+            a.N3 = N3;
+            a.A1 = A1;
+            a.A2 = A2;
+            a.nf = 4;
+            a.in[0] = mf(SF_U1);
+            a.in[1] = mf(SF_U2);
+            a.in[2] = mf(SF_U3);
+            a.in[3] = mf(SF_B);
+            for (int i = 0; i < 4; ++i) a.out[i] = Y[i];
+            a.tw = tw2;
+            a.scale = T(1);
+            check(launch_yinv<T>(a, N2, stream), "yinv");
+            if (linear) {
+                if (!Yb[0])
+                    for (int i = 0; i < 4; ++i) Yb[i] = dalloc<V>(size_t(N2) * A1 * N3);
+                a.in[0] = mf(SF_U1_TOT);
+                a.in[1] = mf(SF_U2_TOT);
+                a.in[2] = mf(SF_U3_TOT);
+                a.in[3] = mf(SF_B_TOT);
+                for (int i = 0; i < 4; ++i) a.out[i] = Yb[i];
+                check(launch_yinv<T>(a, N2, stream), "yinv(base)");
+            }
+            XArgs<T> x{};
+            x.N3 = N3;
+            x.N3p = N3p;
+            x.A1 = A1;
+            x.in_n2_stride = size_t(A1) * N3;
+            x.out_n2_stride = size_t(A1) * N3;
+            for (int i = 0; i < 4; ++i) x.in[i] = Y[i];
+            for (int i = 0; i < 8; ++i) x.out[i] = X[i];
+            x.zc = zc;
+            x.kx = kx;
+            x.tw = tw1;
+            x.scale = T(1);
+            x.linear = linear ? 1 : 0;
+            for (int i = 0; i < 4; ++i) x.base[i] = linear ? Yb[i] : nullptr;
+            check(launch_xpass<T>(x, N1, N2, true, stream), "xpass");
+            YArgs<T> f{};
+            f.N3 = N3;
+            f.A1 = A1;
+            f.A2 = A2;
+            f.nf = 4;
+            f.paired = 1;
+            for (int r = 0; r < 4; ++r) {
+                f.in[2 * r] = X[r];
+                f.in[2 * r + 1] = X[4 + r];
+                f.out[r] = nl[r];
+            }
+            f.ky = ky;
+            f.tw = tw2;
+            f.scale = T(1.0 / (double(N1) * double(N2)));
+            check(launch_yfwd<T>(f, N2, stream), "yfwd");
+        } else {
+            XArgs<T> x{};
+            x.N3 = N3;
+            x.N3p = N3p;
+            x.A1 = A1;
+            x.in_n2_stride = 0;
+            x.out_n2_stride = 0;
+            x.in[0] = mf(SF_U1);
+            x.in[1] = nullptr;
+            x.in[2] = mf(SF_U3);
+            x.in[3] = mf(SF_B);
+            x.out[0] = nl[0];
+            x.out[1] = nl[1];
+            x.out[2] = nl[2];
+            x.out[3] = nl[3];
+            for (int i = 4; i < 8; ++i) x.out[i] = nullptr;
+            x.zc = zc;
+            x.kx = kx;
+            x.tw = tw1;
+            x.scale = T(1.0 / double(N1));
+            x.linear = linear ? 1 : 0;
+            x.base[0] = mf(SF_U1_TOT);
+            x.base[1] = nullptr;
+            x.base[2] = mf(SF_U3_TOT);
+            x.base[3] = mf(SF_B_TOT);
+            check(launch_xpass<T>(x, N1, 1, false, stream), "xpass");
+        }
+    }
+
+    ModalArgs<T> modal_args() {
+        ModalArgs<T> m{};
+        m.N3 = N3;
+        m.N3p = N3p;
+        m.A1 = A1;
+        m.A2 = A2;
+        m.nmodes = A1 * A2;
+        m.threeD = three ? 1 : 0;
+        m.zc = zc;
+        m.kx = kx;
+        m.ky = ky;
+        m.kx2 = kx2;
+        m.ky2 = ky2;
+        m.u1 = mf(SF_U1);
+        m.u2 = three ? mf(SF_U2) : nullptr;
+        m.u3 = mf(SF_U3);
+        m.b = mf(SF_B);
+        m.p = mf(SF_P);
+        return m;
+    }
+
+    // one stage k; IMEXRK.cpp:8-24 (and :26-41 when linear)
+    void stage(int k, bool linear, bool store_r) {
+        static const double beta[3] = {1.0, 25.0 / 8.0, 9.0 / 4.0};    // IMEXRK.h:631
+        static const double zeta[3] = {0.0, -17.0 / 8.0, -5.0 / 4.0};  // IMEXRK.h:632
+        transform_path(linear);
+        ModalArgs<T> m = modal_args();
+        const int nxt = 1 - rcur;
+        const bool read_old = (k > 0);
+        m.ro1 = read_old ? rbuf[rcur][0] : nullptr;
+        m.ro2 = (read_old && three) ? rbuf[rcur][1] : nullptr;
+        m.ro3 = read_old ? rbuf[rcur][2] : nullptr;
+        m.roB = read_old ? rbuf[rcur][3] : nullptr;
+        m.rn1 = store_r ? rbuf[nxt][0] : nullptr;
+        m.rn2 = (store_r && three) ? rbuf[nxt][1] : nullptr;
+        m.rn3 = store_r ? rbuf[nxt][2] : nullptr;
+        m.rnB = store_r ? rbuf[nxt][3] : nullptr;
+        m.n1 = nl[0];
+        m.n2 = three ? nl[1] : nullptr;
+        m.n3 = nl[2];
+        m.nB = nl[3];
+        m.hz = h[k] * T(zeta[k]);
+        m.mh = -h[k];
+        m.hb = h[k] * T(beta[k]);
+        m.cv = (T(0.5) * h[k]) / T(Re);
+        m.cb = ((T(0.5) * h[k]) / T(Re)) / T(Pr);
+        m.Ri = T(Ri);
+        m.pmult = T(1) / h[k];
+        m.mode = 0;
+        check(launch_modal<T>(m, stream), "modal");
+        if (store_r) {
+            rcur = nxt;
+            bind_r();
+        }
+    }
+
+    void step(int n, bool linear) {
+        for (int s = 0; s < n; ++s)
+            for (int k = 0; k < 3; ++k) stage(k, linear, k < 2);
+    }
+
+    void remove_divergence(double mult) {
+        ModalArgs<T> m = modal_args();
+        m.pmult = T(mult);
+        m.mode = 1;
+        check(launch_modal<T>(m, stream), "remove_divergence");
+    }
+
+    // ModalField::ToNodal (Field.h:965-994)
+    void to_nodal(const V* modal, T* nodal) {
+        XTArgs<T> x{};
+        x.N3 = N3;
+        x.A1 = A1;
+        x.tw = tw1;
+        x.nodal_out = nodal;
+        x.scale = T(1);
+        if (three) {
+            YArgs<T> a{};
+            a.N3 = N3;
+            a.A1 = A1;
+            a.A2 = A2;
+            a.nf = 1;
+            a.in[0] = modal;
+            a.out[0] = X[0];
+            a.tw = tw2;
+            a.scale = T(1);
+            check(launch_yinv<T>(a, N2, stream), "to_nodal:y");
+            x.spec_in = X[0];
+            x.spec_n2_stride = size_t(A1) * N3;
+        } else {
+            x.spec_in = modal;
+            x.spec_n2_stride = 0;
+        }
+        check(launch_x_c2r<T>(x, N1, N2, stream), "to_nodal:x");
+    }
+
+    // NodalField::ToModal (Field.h:734-769): scale 1/(N1*N2), keep active modes
+    void to_modal(const T* nodal, V* modal) {
+        XTArgs<T> x{};
+        x.N3 = N3;
+        x.A1 = A1;
+        x.tw = tw1;
+        x.nodal_in = nodal;
+        if (three) {
+            x.spec_out = X[0];
+            x.spec_n2_stride = size_t(A1) * N3;
+            x.scale = T(1);
+            check(launch_x_r2c<T>(x, N1, N2, stream), "to_modal:x");
+            YArgs<T> a{};
+            a.N3 = N3;
+            a.A1 = A1;
+            a.A2 = A2;
+            a.nf = 1;
+            a.paired = 0;
+            a.in[0] = X[0];
+            a.out[0] = modal;
+            a.tw = tw2;
+            a.ky = ky;
+            a.scale = T(1.0 / (double(N1) * double(N2)));
+            check(launch_yfwd<T>(a, N2, stream), "to_modal:y");
+        } else {
+            x.spec_out = modal;
+            x.spec_n2_stride = 0;
+            x.scale = T(1.0 / double(N1));
+            check(launch_x_r2c<T>(x, N1, N2, stream), "to_modal:x");
+        }
+    }
+
+    void populate_nodal() {
+        to_nodal(mf(SF_U1), static_cast<T*>(state[SF_NODAL_U1].d));
+        if (three) to_nodal(mf(SF_U2), static_cast<T*>(state[SF_NODAL_U2].d));
+        to_nodal(mf(SF_U3), static_cast<T*>(state[SF_NODAL_U3].d));
+        to_nodal(mf(SF_B), static_cast<T*>(state[SF_NODAL_B].d));
+    }
+
+    // dot products into scal[slot]
+    void dot_async(const V* a, const V* b, int bc, int slot) {
+        const T* w = zc + size_t(bc == SF_NEUMANN ? ZC_WN : ZC_WD) * N3p;
+        check(launch_dot<T>(a, b, A1 * A2, N3, w, partial, scal, slot, stream), "dot");
+    }
+    void read_scalars(int n, double* out) {
+        SF_HIP(hipMemcpyAsync(scal_host, scal, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+        for (int i = 0; i < n; ++i) out[i] = scal_host[i];
+    }
+
+    void energy(double* ke, double* pe) {
+        dot_async(mf(SF_U1), mf(SF_U1), SF_NEUMANN, 0);
+        dot_async(mf(SF_U3), mf(SF_U3), SF_DIRICHLET, 1);
+        if (three) dot_async(mf(SF_U2), mf(SF_U2), SF_NEUMANN, 2);
+        dot_async(mf(SF_B), mf(SF_B), SF_NEUMANN, 3);
+        double s[4];
+        read_scalars(4, s);
+        double e = 0.5 * (s[0] + s[1]);  // IMEXRK.h:395-406
+        if (three) e += 0.5 * s[2];
+        *ke = e;
+        *pe = Ri * 0.5 * s[3];  // IMEXRK.h:408-411
+    }
+
+    // ---- host <-> device with layout conversion ----
+    void upload(sf_field* f, const double* host) {
+        if (f->kind == SF_MODAL) {
+            std::vector<V> h(nmodal);
+            for (int ja = 0; ja < A2; ++ja) {
+                const int j2 = three ? ((ja < N2 / 3) ? ja : ja + N2 - 2 * (N2 / 3) + 1) : 0;
+                for (int j1 = 0; j1 < A1; ++j1)
+                    for (int j3 = 0; j3 < N3; ++j3) {
+                        const size_t src = (size_t(M1) * j2 + j1) * N3 + j3;
+                        const size_t dst = (size_t(A1) * ja + j1) * N3 + j3;
+                        h[dst].x = T(host[2 * src]);
+                        h[dst].y = T(host[2 * src + 1]);
+                    }
+            }
+            SF_HIP(hipMemcpyAsync(f->d, h.data(), nmodal * sizeof(V), hipMemcpyHostToDevice, stream));
+            SF_HIP(hipStreamSynchronize(stream));
+        } else {
+            std::vector<T> h(f->n);
+            for (size_t i = 0; i < f->n; ++i) h[i] = T(host[i]);
+            SF_HIP(hipMemcpyAsync(f->d, h.data(), f->n * sizeof(T), hipMemcpyHostToDevice, stream));
+            SF_HIP(hipStreamSynchronize(stream));
+        }
+    }
+    void download(const sf_field* f, double* host) {
+        if (f->kind == SF_MODAL) {
+            std::vector<V> h(nmodal);
+            SF_HIP(hipMemcpyAsync(h.data(), f->d, nmodal * sizeof(V), hipMemcpyDeviceToHost, stream));
+            SF_HIP(hipStreamSynchronize(stream));
+            const size_t full = size_t(M1) * N2 * N3;
+            std::memset(host, 0, 2 * full * sizeof(double));
+            for (int ja = 0; ja < A2; ++ja) {
+                const int j2 = three ? ((ja < N2 / 3) ? ja : ja + N2 - 2 * (N2 / 3) + 1) : 0;
+                for (int j1 = 0; j1 < A1; ++j1)
+                    for (int j3 = 0; j3 < N3; ++j3) {
+                        const size_t dst = (size_t(M1) * j2 + j1) * N3 + j3;
+                        const size_t src = (size_t(A1) * ja + j1) * N3 + j3;
+                        host[2 * dst] = h[src].x;
+                        host[2 * dst + 1] = h[src].y;
+                    }
+            }
+        } else {
+            std::vector<T> h(f->n);
+            SF_HIP(hipMemcpyAsync(h.data(), f->d, f->n * sizeof(T), hipMemcpyDeviceToHost, stream));
+            SF_HIP(hipStreamSynchronize(stream));
+            for (size_t i = 0; i < f->n; ++i) host[i] = h[i];
+        }
+    }
+
+    sf_field* new_field(int kind, int bc) {
+        SF_REQUIRE(kind == SF_NODAL || kind == SF_MODAL || kind == SF_1D, SF_ERR_ARG, "bad field kind");
+        SF_REQUIRE(bc == SF_NEUMANN || bc == SF_DIRICHLET, SF_ERR_ARG, "bad boundary condition");
+        std::unique_ptr<sf_field> f(new sf_field());
+        f->ctx = owner;
+        f->kind = kind;
+        f->bc = bc;
+        f->owned = true;
+        f->n = kind == SF_MODAL ? 2 * nmodal : (kind == SF_NODAL ? nnodal : size_t(N3));
+        void* p = nullptr;
+        SF_HIP(hipMalloc(&p, f->n * sizeof(T)));
+        f->d = p;
+        SF_HIP(hipMemsetAsync(p, 0, f->n * sizeof(T), stream));
+        return f.release();
+    }
+};
+
+template <typename F>
+static int guarded(sf_ctx* c, F fn) {
+    try {
+        fn();
+        if (c) c->err.clear();
+        return SF_OK;
+    } catch (const Error& e) {
+        if (c) c->err = e.what();
+        else g_create_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (c) c->err = "host allocation failed";
+        return SF_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        return SF_ERR_STATE;
+    }
+}
+
+template <typename Fn>
+static void with_ctx(sf_ctx* c, Fn fn) {
+    if (c->precision == 4) fn(*static_cast<Ctx<float>*>(c->raw));
+    else fn(*static_cast<Ctx<double>*>(c->raw));
+}
+
+}  // namespace sf
+
+using namespace sf;
+
+#define SF_NULLCHK(p)                                                  \
+    do {                                                               \
+        if (!(p)) return SF_ERR_ARG;                                   \
+    } while (0)
+
+extern "C" {
+
+int sf_ctx_create(const sf_grid* grid, const sf_flow* flow, int precision, sf_ctx** out) {
+    if (!grid || !flow || !out) {
+        g_create_error = "null argument";
+        return SF_ERR_ARG;
+    }
+    *out = nullptr;
+    std::unique_ptr<sf_ctx> c(new sf_ctx());
+    c->precision = precision == 8 ? 8 : 4;
+    if (precision != 4 && precision != 8) {
+        g_create_error = "precision must be 4 or 8";
+        return SF_ERR_ARG;
+    }
+    const int rc = guarded(nullptr, [&]() {
+        if (c->precision == 4) {
+            auto p = std::make_shared<Ctx<float>>();
+            p->owner = c.get();
+            p->init(*grid, *flow);
+            c->raw = p.get();
+            c->impl = p;
+        } else {
+            auto p = std::make_shared<Ctx<double>>();
+            p->owner = c.get();
+            p->init(*grid, *flow);
+            c->raw = p.get();
+            c->impl = p;
+        }
+    });
+    if (rc != SF_OK) return rc;
+    *out = c.release();
+    return SF_OK;
+}
+
+int sf_ctx_destroy(sf_ctx* ctx) {
+    delete ctx;
+    return SF_OK;
+}
+
+const char* sf_last_error(const sf_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int sf_set_dt(sf_ctx* ctx, double dt) {
+    SF_NULLCHK(ctx);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { C.set_dt(dt); }); });
+}
+
+int sf_get_dt(const sf_ctx* ctx, double* dt) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(dt);
+    with_ctx(const_cast<sf_ctx*>(ctx), [&](auto& C) { *dt = C.dt; });
+    return SF_OK;
+}
+
+int sf_set_flow(sf_ctx* ctx, const sf_flow* flow) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(flow);
+    return guarded(ctx, [&]() {
+        with_ctx(ctx, [&](auto& C) {
+            SF_REQUIRE(!flow->evolve_background, SF_ERR_UNSUPPORTED, "EvolveBackground is not supported");
+            C.set_flow(*flow);
+        });
+    });
+}
+
+int sf_set_background(sf_ctx* ctx, const double* U, const double* dBdz) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(U);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { C.set_background(U, dBdz); }); });
+}
+
+int sf_vertical_points(const sf_ctx* ctx, double* z, double* zf) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(z);
+    SF_NULLCHK(zf);
+    with_ctx(const_cast<sf_ctx*>(ctx), [&](auto& C) {
+        const VGrid g = vertical_grid(C.L3, C.N3);
+        for (int j = 0; j < C.N3; ++j) {
+            z[j] = g.z[j];
+            zf[j] = g.zf[j];
+        }
+    });
+    return SF_OK;
+}
+
+int sf_field_alloc(sf_ctx* ctx, int kind, int bc, sf_field** out) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(out);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { *out = C.new_field(kind, bc); }); });
+}
+
+int sf_field_free(sf_field* f) {
+    if (!f) return SF_OK;
+    if (!f->owned) return SF_ERR_ARG;
+    sf_synchronize(f->ctx);
+    (void)hipFree(f->d);
+    delete f;
+    return SF_OK;
+}
+
+int sf_field_kind(const sf_field* f, int* kind, int* bc) {
+    SF_NULLCHK(f);
+    if (kind) *kind = f->kind;
+    if (bc) *bc = f->bc;
+    return SF_OK;
+}
+
+int sf_field_upload(sf_field* f, const double* host) {
+    SF_NULLCHK(f);
+    SF_NULLCHK(host);
+    return guarded(f->ctx, [&]() { with_ctx(f->ctx, [&](auto& C) { C.upload(f, host); }); });
+}
+
+int sf_field_download(const sf_field* f, double* host) {
+    SF_NULLCHK(f);
+    SF_NULLCHK(host);
+    return guarded(f->ctx, [&]() { with_ctx(f->ctx, [&](auto& C) { C.download(f, host); }); });
+}
+
+static bool same_shape(const sf_field* a, const sf_field* b) {
+    return a->ctx == b->ctx && a->kind == b->kind && a->n == b->n;
+}
+
+int sf_field_copy(sf_field* dst, const sf_field* src) {
+    SF_NULLCHK(dst);
+    SF_NULLCHK(src);
+    return guarded(dst->ctx, [&]() {
+        SF_REQUIRE(same_shape(dst, src) && dst->bc == src->bc, SF_ERR_ARG, "field shape/BC mismatch (Field.h:285)");
+        with_ctx(dst->ctx, [&](auto& C) {
+            using TT = typename std::remove_reference<decltype(C)>::type;
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            (void)sizeof(TT);
+            SF_HIP(hipMemcpyAsync(dst->d, src->d, dst->n * sizeof(T), hipMemcpyDeviceToDevice, C.stream));
+        });
+    });
+}
+
+int sf_field_zero(sf_field* f) {
+    SF_NULLCHK(f);
+    return guarded(f->ctx, [&]() {
+        with_ctx(f->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            SF_HIP(hipMemsetAsync(f->d, 0, f->n * sizeof(T), C.stream));
+        });
+    });
+}
+
+int sf_field_axpby(sf_field* y, double a, const sf_field* x, double b) {
+    SF_NULLCHK(y);
+    return guarded(y->ctx, [&]() {
+        if (x) SF_REQUIRE(same_shape(x, y) && x->bc == y->bc, SF_ERR_ARG, "field shape/BC mismatch (Field.h:352)");
+        with_ctx(y->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            C.check(launch_axpby<T>(static_cast<cx<T>*>(y->d), T(a), x ? static_cast<const cx<T>*>(x->d) : nullptr,
+                                    T(b), y->n / 2, C.stream),
+                    "axpby");
+        });
+    });
+}
+
+int sf_field_scale(sf_field* f, double a) { return sf_field_axpby(f, 0.0, nullptr, a); }
+
+static int ends_op(sf_field* f, int op) {
+    SF_NULLCHK(f);
+    return guarded(f->ctx, [&]() {
+        SF_REQUIRE(f->kind == SF_MODAL, SF_ERR_UNSUPPORTED, "end conditions are implemented for modal fields");
+        with_ctx(f->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            C.check(launch_ends<T>(static_cast<cx<T>*>(f->d), C.A1 * C.A2, C.N3, op, C.stream), "ends");
+        });
+    });
+}
+
+int sf_field_zero_ends(sf_field* f) { return ends_op(f, f && f->bc == SF_DIRICHLET ? 1 : 0); }
+
+int sf_field_neumann_ends(sf_field* f) {
+    if (f && f->bc != SF_NEUMANN) {
+        f->ctx->err = "NeumannEnds requires a Neumann field (Field.h:493-499)";
+        return SF_ERR_ARG;
+    }
+    return ends_op(f, 2);
+}
+
+int sf_field_filter(sf_field* f) {
+    SF_NULLCHK(f);
+    return SF_OK;
+}
+
+int sf_to_modal(const sf_field* nodal, sf_field* modal) {
+    SF_NULLCHK(nodal);
+    SF_NULLCHK(modal);
+    return guarded(modal->ctx, [&]() {
+        SF_REQUIRE(nodal->kind == SF_NODAL && modal->kind == SF_MODAL && nodal->bc == modal->bc, SF_ERR_ARG,
+                   "ToModal needs a nodal input and a modal output with the same BC (Field.h:736)");
+        with_ctx(modal->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            C.to_modal(static_cast<const T*>(nodal->d), static_cast<cx<T>*>(modal->d));
+        });
+    });
+}
+
+int sf_to_nodal(const sf_field* modal, sf_field* nodal) {
+    SF_NULLCHK(nodal);
+    SF_NULLCHK(modal);
+    return guarded(modal->ctx, [&]() {
+        SF_REQUIRE(nodal->kind == SF_NODAL && modal->kind == SF_MODAL && nodal->bc == modal->bc, SF_ERR_ARG,
+                   "ToNodal needs a modal input and a nodal output with the same BC (Field.h:967)");
+        with_ctx(modal->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            C.to_nodal(static_cast<const cx<T>*>(modal->d), static_cast<T*>(nodal->d));
+        });
+    });
+}
+
+static int deriv(const sf_field* in, sf_field* out, int axis) {
+    SF_NULLCHK(in);
+    SF_NULLCHK(out);
+    return guarded(out->ctx, [&]() {
+        SF_REQUIRE(in->kind == SF_MODAL && out->kind == SF_MODAL && in->d != out->d, SF_ERR_ARG,
+                   "derivatives act on distinct modal fields");
+        int ax = axis;
+        if (axis == 2) {
+            SF_REQUIRE(out->bc != in->bc, SF_ERR_ARG, "ddz changes the boundary condition (Stratiflow.h:67-80)");
+            ax = in->bc == SF_NEUMANN ? 2 : 3;
+        } else {
+            SF_REQUIRE(out->bc == in->bc, SF_ERR_ARG, "ddx/ddy keep the boundary condition");
+        }
+        with_ctx(out->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            C.check(launch_dmul<T>(static_cast<cx<T>*>(out->d), static_cast<const cx<T>*>(in->d), C.A1, C.A2, C.N3,
+                                   C.kx, C.ky, ax, C.zc, C.N3p, C.stream),
+                    "deriv");
+        });
+    });
+}
+
+int sf_ddx(const sf_field* in, sf_field* out) { return deriv(in, out, 0); }
+int sf_ddy(const sf_field* in, sf_field* out) { return deriv(in, out, 1); }
+int sf_ddz(const sf_field* in, sf_field* out) { return deriv(in, out, 2); }
+
+int sf_inner_prod(const sf_field* a, const sf_field* b, double* out) {
+    SF_NULLCHK(a);
+    SF_NULLCHK(b);
+    SF_NULLCHK(out);
+    return guarded(a->ctx, [&]() {
+        SF_REQUIRE(a->kind == SF_MODAL && same_shape(a, b) && a->bc == b->bc, SF_ERR_ARG,
+                   "InnerProd needs two modal fields with the same BC (Integration.h:285)");
+        with_ctx(a->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            C.dot_async(static_cast<const cx<T>*>(a->d), static_cast<const cx<T>*>(b->d), a->bc, 0);
+            C.read_scalars(1, out);
+        });
+    });
+}
+
+int sf_state_field(sf_ctx* ctx, int which, sf_field** out) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(out);
+    return guarded(ctx, [&]() {
+        SF_REQUIRE(which >= 0 && which <= SF_B_TOT, SF_ERR_ARG, "bad state field id");
+        with_ctx(ctx, [&](auto& C) { *out = &C.state[which]; });
+    });
+}
+
+int sf_step(sf_ctx* ctx, int nsteps) {
+    SF_NULLCHK(ctx);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { C.step(nsteps, false); }); });
+}
+
+int sf_step_linear(sf_ctx* ctx, int nsteps) {
+    SF_NULLCHK(ctx);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { C.step(nsteps, true); }); });
+}
+
+int sf_set_linear_background(sf_ctx* ctx, const sf_field* u1, const sf_field* u2, const sf_field* u3,
+                             const sf_field* b) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(u1);
+    SF_NULLCHK(u3);
+    SF_NULLCHK(b);
+    return guarded(ctx, [&]() {
+        with_ctx(ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            const sf_field* src[4] = {u1, u2, u3, b};
+            for (int i = 0; i < 4; ++i) {
+                sf_field& d = C.state[SF_U1_TOT + i];
+                if (!src[i]) {
+                    SF_HIP(hipMemsetAsync(d.d, 0, d.n * sizeof(T), C.stream));
+                    continue;
+                }
+                SF_REQUIRE(src[i]->kind == SF_MODAL && src[i]->n == d.n, SF_ERR_ARG, "background must be modal");
+                SF_HIP(hipMemcpyAsync(d.d, src[i]->d, d.n * sizeof(T), hipMemcpyDeviceToDevice, C.stream));
+            }
+        });
+    });
+}
+
+int sf_stage(sf_ctx* ctx, int k, int linear) {
+    SF_NULLCHK(ctx);
+    return guarded(ctx, [&]() {
+        SF_REQUIRE(k >= 0 && k < 3, SF_ERR_ARG, "stage must be 0, 1 or 2");
+        with_ctx(ctx, [&](auto& C) { C.stage(k, linear != 0, true); });
+    });
+}
+
+int sf_remove_divergence(sf_ctx* ctx, double mult) {
+    SF_NULLCHK(ctx);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { C.remove_divergence(mult); }); });
+}
+
+int sf_filter_all(sf_ctx* ctx) {
+    SF_NULLCHK(ctx);
+    return SF_OK;  // compact storage holds active modes only
+}
+
+int sf_populate_nodal(sf_ctx* ctx) {
+    SF_NULLCHK(ctx);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { C.populate_nodal(); }); });
+}
+
+int sf_energy(sf_ctx* ctx, double* ke, double* pe) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(ke);
+    SF_NULLCHK(pe);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { C.energy(ke, pe); }); });
+}
+
+int sf_cfl(sf_ctx* ctx, double* cfl) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(cfl);
+    ctx->err = "sf_cfl is not implemented yet";
+    return SF_ERR_UNSUPPORTED;
+}
+
+int sf_synchronize(sf_ctx* ctx) {
+    SF_NULLCHK(ctx);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { SF_HIP(hipStreamSynchronize(C.stream)); }); });
+}
+
+int sf_time_steps(sf_ctx* ctx, int nsteps, double* ms) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(ms);
+    return guarded(ctx, [&]() {
+        with_ctx(ctx, [&](auto& C) {
+            SF_HIP(hipEventRecord(C.ev0, C.stream));
+            C.step(nsteps, false);
+            SF_HIP(hipEventRecord(C.ev1, C.stream));
+            SF_HIP(hipEventSynchronize(C.ev1));
+            float t = 0;
+            SF_HIP(hipEventElapsedTime(&t, C.ev0, C.ev1));
+            *ms = t;
+        });
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// sf_common.h -- shared types and small device helpers for the gfx950 path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace sf {
+
+template <typename T>
+struct CxT;
+template <>
+struct CxT<float> {
+    using t = float2;
+};
+template <>
+struct CxT<double> {
+    using t = double2;
+};
+template <typename T>
+using cx = typename CxT<T>::t;
+
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> mkc(T r, T i) {
+    cx<T> c;
+    c.x = r;
+    c.y = i;
+    return c;
+}
+template <typename V>
+__host__ __device__ __forceinline__ V cadd(V a, V b) {
+    V r;
+    r.x = a.x + b.x;
+    r.y = a.y + b.y;
+    return r;
+}
+template <typename V>
+__host__ __device__ __forceinline__ V csub(V a, V b) {
+    V r;
+    r.x = a.x - b.x;
+    r.y = a.y - b.y;
+    return r;
+}
+template <typename V>
+__host__ __device__ __forceinline__ V cmul(V a, V b) {
+    V r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+// a * conj(b)
+template <typename V>
+__host__ __device__ __forceinline__ V cmulc(V a, V b) {
+    V r;
+    r.x = a.x * b.x + a.y * b.y;
+    r.y = a.y * b.x - a.x * b.y;
+    return r;
+}
+template <typename V>
+__host__ __device__ __forceinline__ V cconj(V a) {
+    V r;
+    r.x = a.x;
+    r.y = -a.y;
+    return r;
+}
+template <typename V, typename T>
+__host__ __device__ __forceinline__ V cscale(T s, V a) {
+    V r;
+    r.x = s * a.x;
+    r.y = s * a.y;
+    return r;
+}
+// (i*k) * a  for real k
+template <typename V, typename T>
+__host__ __device__ __forceinline__ V cik(T k, V a) {
+    V r;
+    r.x = -k * a.y;
+    r.y = k * a.x;
+    return r;
+}
+// a + s*b
+template <typename V, typename T>
+__host__ __device__ __forceinline__ V caxpy(T s, V b, V a) {
+    V r;
+    r.x = a.x + s * b.x;
+    r.y = a.y + s * b.y;
+    return r;
+}
+template <typename V>
+__host__ __device__ __forceinline__ V czero() {
+    V r;
+    r.x = 0;
+    r.y = 0;
+    return r;
+}
+
+// Per-z coefficient table rows (see sf_context.cpp for their definition).
+enum ZCoef {
+    ZC_DZZN_LO = 0,
+    ZC_DZZN_DI,
+    ZC_DZZN_UP,
+    ZC_DZZD_LO,
+    ZC_DZZD_DI,
+    ZC_DZZD_UP,
+    ZC_DZND,   // 1/DY_j   rows 2..N-1  (Neumann -> Dirichlet derivative)
+    ZC_DZDN,   // 1/DYF_j  rows 1..N-1  (Dirichlet -> Neumann derivative)
+    ZC_IFULL_LO,
+    ZC_IFULL_DI,
+    ZC_ITIL_LO,
+    ZC_ITIL_DI,
+    ZC_UBAR,   // background velocity U_(z)
+    ZC_S,      // background stratification dB/dz at Neumann points
+    ZC_WN,     // InnerProd weight, Neumann
+    ZC_WD,     // InnerProd weight, Dirichlet
+    ZC_COUNT
+};
+
+}  // namespace sf

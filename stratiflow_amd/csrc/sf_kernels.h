@@ -1,0 +1,87 @@
+// sf_kernels.h -- kernel argument blocks and host launchers.
+#pragma once
+#include "sf_common.h"
+
+namespace sf {
+
+// Fused per-mode stage kernel (one wavefront per Fourier mode).
+template <typename T>
+struct ModalArgs {
+    int N3, N3p, A1, A2, nmodes, threeD;
+    const T* zc;  // [ZC_COUNT][N3p]
+    const T *kx, *ky, *kx2, *ky2;
+    cx<T> *u1, *u2, *u3, *b, *p;
+    const cx<T> *ro1, *ro2, *ro3, *roB;  // explicit terms of the previous stage (null: not read)
+    cx<T> *rn1, *rn2, *rn3, *rnB;        // explicit terms of this stage (null: not stored)
+    const cx<T> *n1, *n2, *n3, *nB;      // nonlinear terms from the transform path
+    T hz, mh, hb, cv, cb, Ri, pmult;
+    int mode;  // 0: full stage, 1: projection only (RemoveDivergence)
+};
+
+// Fused inverse-x / products / forward-x kernel.
+template <typename T>
+struct XArgs {
+    int N3, N3p, A1, nzt;
+    size_t in_n2_stride, out_n2_stride;  // elements between consecutive n2 planes
+    const cx<T>* in[4];                  // u1, u2, u3, b  ([n2][j1][z])
+    cx<T>* out[8];                       // 3D: A1,A2,A3,AB,B1,B2,B3,BB ; 2D: Nl1,-,Nl3,NlB
+    const T* zc;
+    const T* kx;
+    const cx<T>* tw;
+    T scale;
+    int linear;                          // 1: linearised products (BuildRHSLinear)
+    const cx<T>* base[4];                // linear: u1t, u2t, u3t, bt spectra ([n2][j1][z])
+};
+
+// y-direction transforms (3D): per (j1, z-tile).
+template <typename T>
+struct YArgs {
+    int N3, A1, A2, nzt, nf;
+    const cx<T>* in[8];
+    cx<T>* out[8];
+    const T* ky;      // A2 active wavenumbers (forward: i*ky*B term)
+    const cx<T>* tw;
+    T scale;
+    int paired;       // forward: out[r] = scale*(F(in[2r]) + i ky F(in[2r+1]))
+};
+
+// Single-field x transforms between nodal and x-spectral layouts.
+template <typename T>
+struct XTArgs {
+    int N3, A1, nzt;
+    size_t spec_n2_stride;
+    const cx<T>* spec_in;
+    cx<T>* spec_out;
+    const T* nodal_in;
+    T* nodal_out;
+    const cx<T>* tw;
+    T scale;
+};
+
+template <typename T>
+hipError_t launch_modal(const ModalArgs<T>& a, hipStream_t s);
+template <typename T>
+hipError_t launch_xpass(const XArgs<T>& a, int N1, int N2, bool threeD, hipStream_t s);
+template <typename T>
+hipError_t launch_yinv(const YArgs<T>& a, int N2, hipStream_t s);
+template <typename T>
+hipError_t launch_yfwd(const YArgs<T>& a, int N2, hipStream_t s);
+template <typename T>
+hipError_t launch_x_c2r(const XTArgs<T>& a, int N1, int N2, hipStream_t s);
+template <typename T>
+hipError_t launch_x_r2c(const XTArgs<T>& a, int N1, int N2, hipStream_t s);
+template <typename T>
+hipError_t launch_dot(const cx<T>* a, const cx<T>* b, int nmodes, int N3, const T* w, double* partial,
+                      double* out, int slot, hipStream_t s);
+template <typename T>
+hipError_t launch_axpby(cx<T>* y, T a, const cx<T>* x, T b, size_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_ends(cx<T>* f, int nstacks, int N3, int bc_op, hipStream_t s);
+template <typename T>
+hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, const T* kx, const T* ky, int axis,
+                       const T* zc, int N3p, hipStream_t s);
+
+bool tz_supported(int n);
+int max_rows_per_lane();
+
+}  // namespace sf

@@ -1,0 +1,89 @@
+"""GPU path vs CPU oracle on identical seeded inputs (calls go through the C ABI)."""
+import numpy as np
+import pytest
+
+from helpers import fields, make_pair, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CASES_2D = [(48, 1, 64), (128, 1, 128)]
+CASES_3D = [(24, 12, 32), (48, 24, 64)]
+
+
+def _report(tag, **kv):
+    print(tag, " ".join("%s=%.3e" % kv_ for kv_ in kv.items()), flush=True)
+
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-11), (4, 2e-5)])
+@pytest.mark.parametrize("shape", [(48, 1, 64, 1), (24, 12, 32, 0)])
+def test_transforms_roundtrip(shape, prec, tol):
+    """ToNodal / ToModal against the oracle's FFT on the same spectra."""
+    N1, N2, N3, dim = shape
+    s, o = make_pair(N1, N2, N3, dim, gpu_prec=prec, project=False)
+    import stratiflow_amd as sf
+    u = s.state("u1").download()
+    nod = s.field(sf.NODAL, sf.NEUMANN)
+    s.state("u1").ToNodal(nod)
+    g_nod = nod.download()
+    o_nod = o.to_nodal(u)
+    assert rel_l2(g_nod, o_nod) < tol
+    back = s.field(sf.MODAL, sf.NEUMANN)
+    nod.ToModal(back)
+    g_back = back.download()
+    o_back = o.to_modal(o_nod)
+    assert rel_l2(g_back, o_back) < tol
+    assert rel_l2(g_back, u) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-10), (4, 1e-4)])
+@pytest.mark.parametrize("shape", [(48, 1, 64, 1), (24, 12, 32, 0)])
+def test_remove_divergence(shape, prec, tol):
+    N1, N2, N3, dim = shape
+    s, o = make_pair(N1, N2, N3, dim, gpu_prec=prec, project=False)
+    s.RemoveDivergence(0.7)
+    o.remove_divergence(0.7)
+    for f in fields(dim):
+        if f == "p":
+            continue
+        e = rel_l2(s.get(f), o.get_modal(f))
+        _report("remove_divergence", **{f: e})
+        assert e < tol, f
+
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-9), (4, 2e-4)])
+@pytest.mark.parametrize("shape", [(48, 1, 64, 1), (24, 12, 32, 0)])
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_one_stage(shape, k, prec, tol):
+    N1, N2, N3, dim = shape
+    s, o = make_pair(N1, N2, N3, dim, gpu_prec=prec, amplitude=0.05)
+    # bring both to stage k with identical history
+    for kk in range(k + 1):
+        s.Stage(kk)
+        o.stage(kk)
+    names = fields(dim) + (["r1", "r2", "r3", "rB"] if dim == 0 else ["r1", "r3", "rB"])
+    for f in names:
+        e = rel_l2(s.get(f), o.get_modal(f))
+        _report("stage%d" % k, **{f: e})
+        assert e < tol, f
+
+
+@pytest.mark.parametrize("shape", [(48, 1, 64, 1), (128, 1, 128, 1), (24, 12, 32, 0), (48, 24, 64, 0)])
+def test_steps_fp32_energy(shape):
+    """FP32 GPU vs FP32 oracle and FP64 oracle: KE/PE to 1e-4 after N steps."""
+    N1, N2, N3, dim = shape
+    steps = 20
+    s, o = make_pair(N1, N2, N3, dim, gpu_prec=4, oracle_prec=4, amplitude=0.05)
+    _, o64 = make_pair(N1, N2, N3, dim, gpu_prec=4, oracle_prec=8, amplitude=0.05)
+    s.TimeStep(steps)
+    o.step(steps)
+    o64.step(steps)
+    ke, pe = s.energies()
+    ko, po = o.energy()
+    k8, p8 = o64.energy()
+    _report("steps%s" % (shape,), ke_rel=abs(ke - ko) / ko, pe_rel=abs(pe - po) / po,
+            ke_vs64=abs(ke - k8) / k8, pe_vs64=abs(pe - p8) / p8)
+    assert abs(ke - ko) / ko < 1e-4
+    assert abs(pe - po) / po < 1e-4
+    for f in fields(dim):
+        e = rel_l2(s.get(f), o.get_modal(f))
+        assert e < 1e-3, f

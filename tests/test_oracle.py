@@ -1,0 +1,347 @@
+"""Pins for the CPU oracle (oracle/sf_oracle.cpp) that do not use the oracle's own formulas.
+
+The reference ships no tests, fixtures or golden vectors (it has no test
+directory; Catch is vendored but unused), so every function of the oracle is
+checked against mathematics instead:
+  * grid            -- closed-form end conditions and symmetry (Field.cpp:3-31)
+  * operators       -- exactness on linear data, adjoint identities under the
+                       reference's quadrature weights, composition identities
+  * FFT             -- numpy.fft (pocketfft), an independent implementation
+  * Thomas          -- numpy.linalg.solve on the dense matrix
+  * Crank-Nicolson  -- closed-form amplification of a vertically uniform mode
+  * time stepping   -- second-order convergence under dt halving
+  * projection      -- discrete divergence vanishes on interior rows
+  * InnerProd       -- Parseval in 2D; the documented j1=0 double count in 3D
+  * physics         -- Kelvin-Helmholtz growth rate of the tanh shear layer
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+LX = 2 * np.pi / 0.44
+
+
+def make(N1=24, N2=1, N3=32, dim=O.TWO_D, Re=1000.0, Ri=0.1, Pr=1.0, prec=8, L3=10.0):
+    return O.Oracle(N1, N2, N3, dim, LX, LX / 2, L3, Re, Ri, Pr, precision=prec)
+
+
+# ---------------------------------------------------------------- grid
+@pytest.mark.parametrize("N", [16, 64, 384])
+def test_grid_closed_form(N):
+    o = make(N3=N)
+    z, zf, DY, DYF = o.grid()
+    L = 10.0
+    # walls sit on the Neumann points 1 and N-2 (Field.cpp:14-16)
+    assert zf[1] == pytest.approx(-L, abs=1e-12)
+    assert zf[N - 2] == pytest.approx(L, abs=1e-12)
+    # antisymmetric stretched grid
+    assert np.allclose(z[1:], -z[1:][::-1], atol=1e-12)
+    assert np.all(np.diff(z[1:]) > 0) and np.all(DY[1:] > 0) and np.all(DYF > 0)
+    # refinement at the centre: x^7 map is flattest at x = 0
+    assert DYF[N // 2] < DYF[2]
+
+
+# ---------------------------------------------------------------- operators
+def test_operators_exact_on_linear_data():
+    o = make(N3=48)
+    z, zf, DY, DYF = o.grid()
+    a, b = 0.3, -1.7
+    fN, fD = a + b * zf, a + b * z
+    N = len(z)
+    # Neumann -> Dirichlet derivative rows 2..N-1
+    assert np.allclose(o.apply_op("dz_ND", fN)[2:], b)
+    assert np.all(o.apply_op("dz_ND", fN)[:2] == 0)
+    # Dirichlet -> Neumann derivative rows 1..N-2
+    assert np.allclose(o.apply_op("dz_DN", fD)[1:N - 1], b)
+    # full interpolation onto Dirichlet points is exact for linear data on
+    # rows 2..N-2; rows 1 and N-1 use the copied ghost spacings DYF_0 = DYF_1
+    # and DYF_{N-1} = DYF_{N-2} of dzFractional (Field.cpp:79-80)
+    assert np.allclose(o.apply_op("ifull", fN)[2:N - 1], fD[2:N - 1])
+    assert not np.isclose(o.apply_op("ifull", fN)[1], fD[1])
+    # Dirichlet -> Neumann midpoint interpolation is exact for linear data
+    # (row 0 would read the uninitialised VerticalPoints(0), Field.cpp:11)
+    assert np.allclose(o.apply_op("idir", fD)[1:N - 1], fN[1:N - 1])
+    # both second derivatives annihilate linear data
+    assert np.allclose(o.apply_op("dzz_N", fN)[1:N - 1], 0, atol=1e-9)
+    assert np.allclose(o.apply_op("dzz_D", fD)[2:N - 1], 0, atol=1e-9)
+    # tilde and bar interpolations preserve constants
+    assert np.allclose(o.apply_op("itilde", np.ones(N))[2:N - 1], 1)
+    assert np.allclose(o.apply_op("ibar", np.ones(N))[1:], 1)
+
+
+def test_operator_adjoints_and_compositions():
+    o = make(N3=40)
+    z, zf, DY, DYF = o.grid()
+    N = len(z)
+    rng = np.random.default_rng(3)
+    f = np.zeros(N)
+    g = np.zeros(N)
+    f[3:N - 3] = rng.standard_normal(N - 6)
+    g[3:N - 3] = rng.standard_normal(N - 6)
+    # Itilde is the adjoint of Idir under the weights DY (Dirichlet) and DYF (Neumann)
+    lhs = np.sum(DY * g * o.apply_op("itilde", f))
+    rhs = np.sum(DYF * f * o.apply_op("idir", g))
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+    # Dz(N->D) = -Dz(D->N)^* under the same weights
+    lhs = np.sum(DY * g * o.apply_op("dz_ND", f))
+    rhs = -np.sum(DYF * f * o.apply_op("dz_DN", g))
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+    # second derivatives are compositions of the first derivatives
+    full = rng.standard_normal(N)
+    dzzN = o.apply_op("dzz_N", full)
+    comp = o.apply_op("dz_DN", o.apply_op("dz_ND", full))
+    assert np.allclose(dzzN[2:N - 2], comp[2:N - 2], rtol=1e-10, atol=1e-8)
+    dzzD = o.apply_op("dzz_D", full)
+    comp = o.apply_op("dz_ND", o.apply_op("dz_DN", full))
+    assert np.allclose(dzzD[2:N - 2], comp[2:N - 2], rtol=1e-10, atol=1e-8)
+
+
+# ---------------------------------------------------------------- FFT
+def _mask(N1, N2, three):
+    M1 = N1 // 2 + 1
+    m = np.zeros((N2, M1), bool)
+    for j2 in range(N2):
+        for j1 in range(M1):
+            ok = j1 < N1 // 3
+            if three and N2 > 1:
+                ok = ok and (j2 < N2 // 3 or j2 >= N2 - N2 // 3 + 1)
+            m[j2, j1] = ok
+    return m
+
+
+@pytest.mark.parametrize("N1,N2,dim", [(12, 1, O.TWO_D), (48, 1, O.TWO_D), (12, 6, O.THREE_D), (24, 18, O.THREE_D)])
+@pytest.mark.parametrize("prec,tol", [(8, 1e-13), (4, 2e-6)])
+def test_fft_matches_numpy(N1, N2, dim, prec, tol):
+    N3 = 16
+    o = make(N1=N1, N2=N2, N3=N3, dim=dim, prec=prec)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((N2, N1, N3))
+    mask = _mask(N1, N2, dim == O.THREE_D)
+    ref = np.fft.rfft2(x, axes=(0, 1)) / (N1 * N2) * mask[:, :, None]
+    got = o.to_modal(x)
+    assert np.abs(got - ref).max() < tol * np.abs(ref).max()
+    back = o.to_nodal(got)
+    refb = np.fft.irfft2(ref, s=(N2, N1), axes=(0, 1)) * (N1 * N2)
+    assert np.abs(back - refb).max() < tol * np.abs(refb).max() * 10
+
+
+def test_c2r_ignores_imaginary_dc():
+    o = make(N1=12, N2=1, N3=8)
+    m = np.zeros((1, 7, 8), complex)
+    m[0, 0, :] = 1.0 + 5.0j
+    nod = o.to_nodal(m)
+    assert np.allclose(nod, 1.0)
+
+
+# ---------------------------------------------------------------- Thomas
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("j1", [0, 3])
+def test_thomas_matches_dense_solve(kind, j1):
+    N3 = 48
+    o = make(N1=24, N3=N3, Re=300.0, Pr=0.7)
+    o.set_dt(0.05)
+    rng = np.random.default_rng(kind * 10 + j1)
+    d = rng.standard_normal(N3) + 1j * rng.standard_normal(N3)
+    x, (a, b, c) = o.tri_solve(kind, 1, j1, 0, d)
+    A = np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+    ref = np.linalg.solve(A, d)
+    assert np.abs(x - ref).max() < 1e-9 * max(1, np.abs(ref).max())
+
+
+def test_pressure_pin_rows():
+    o = make(N3=32)
+    _, (a, b, c) = o.tri_solve(3, 0, 0, 0, np.ones(32, complex))
+    assert b[0] == 1 and c[0] == 0 and a[1] == 0 and b[1] == 1 and c[1] == 0
+
+
+# ---------------------------------------------------------------- CN amplification
+def _set_state(o, **fields):
+    for name in ("u1", "u2", "u3", "b", "p"):
+        o.set_modal(name, fields.get(name, np.zeros(o.modal_shape, complex)))
+
+
+@pytest.mark.parametrize("j1,Pr", [(1, 1.0), (3, 0.7)])
+def test_crank_nicolson_amplification_closed_form(j1, Pr):
+    """A vertically uniform buoyancy mode decays by prod_k (1-a_k)/(1+a_k), a_k = h_k k^2/(2 Re Pr)."""
+    N1, N3, Re, dt = 24, 32, 50.0, 0.1
+    o = make(N1=N1, N3=N3, Re=Re, Pr=Pr, Ri=0.0)
+    o.set_background(np.zeros(N3))
+    b = np.zeros(o.modal_shape, complex)
+    b[0, j1, :] = 1.0 + 0.5j
+    _set_state(o, b=b)
+    o.set_dt(dt)
+    o.step(1)
+    out = o.get_modal("b")[0, j1]
+    k2 = (2 * np.pi * j1 / LX) ** 2
+    g = 1.0
+    for hk in (dt * 8 / 15, dt * 2 / 15, dt * 5 / 15):
+        a = 0.5 * hk * k2 / (Re * Pr)
+        g *= (1 - a) / (1 + a)
+    interior = slice(3, N3 - 3)
+    assert np.allclose(out[interior], g * (1.0 + 0.5j), rtol=1e-12)
+
+
+# ---------------------------------------------------------------- projection
+@pytest.mark.parametrize("dim,N2", [(O.TWO_D, 1), (O.THREE_D, 12)])
+def test_projection_is_divergence_free(dim, N2):
+    N1, N3 = 24, 32
+    o = make(N1=N1, N2=N2, N3=N3, dim=dim)
+    rng = np.random.default_rng(9)
+    shp = o.modal_shape
+    fields = {n: (rng.standard_normal(shp) + 1j * rng.standard_normal(shp)) for n in ("u1", "u2", "u3")}
+    _set_state(o, **fields)
+    o.filter_all()
+    o.remove_divergence(1.0)
+    u1, u2, u3 = o.get_modal("u1"), o.get_modal("u2"), o.get_modal("u3")
+    kx = 2 * np.pi / LX * np.array([j if j <= N1 // 3 else 0 for j in range(N1 // 2 + 1)])
+    if dim == O.THREE_D:
+        ky = 2 * np.pi / (LX / 2) * np.array([j if j <= N2 // 3 else (j - N2 if j >= N2 - N2 // 3 else 0)
+                                              for j in range(N2)])
+    else:
+        ky = np.zeros(1)
+    mask = _mask(N1, N2, dim == O.THREE_D)
+    worst = 0.0
+    for j2 in range(N2):
+        for j1 in range(N1 // 2 + 1):
+            if not mask[j2, j1]:
+                continue
+            dz = o.apply_op("dz_DN", u3[j2, j1].real) + 1j * o.apply_op("dz_DN", u3[j2, j1].imag)
+            div = 1j * kx[j1] * u1[j2, j1] + 1j * ky[j2] * u2[j2, j1] * (dim == O.THREE_D) + dz
+            worst = max(worst, np.abs(div[2:N3 - 2]).max())
+    assert worst < 1e-9
+
+
+# ---------------------------------------------------------------- InnerProd
+def test_inner_prod_parseval_2d():
+    N1, N3 = 24, 32
+    o = make(N1=N1, N3=N3)
+    z, zf, DY, DYF = o.grid()
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((1, N1, N3))
+    m = o.to_modal(x)
+    xf = o.to_nodal(m)  # the filtered field
+    w = np.zeros(N3)
+    w[1:N3 - 1] = DYF[1:N3 - 1]
+    ref = np.sum(w * np.mean(xf[0] ** 2, axis=0))
+    assert o.inner_prod(m, m, O.NEUMANN) == pytest.approx(ref, rel=1e-12)
+
+
+def test_inner_prod_3d_double_counts_j1_zero():
+    """Integration.h:124-143 weights every mode but (0,0) by 2, including (0, j2 != 0)."""
+    N1, N2, N3 = 12, 6, 16
+    o = make(N1=N1, N2=N2, N3=N3, dim=O.THREE_D)
+    z, zf, DY, DYF = o.grid()
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((N2, N1, N3))
+    m = o.to_modal(x)
+    xf = o.to_nodal(m)
+    w = np.zeros(N3)
+    w[1:N3 - 1] = DYF[1:N3 - 1]
+    parseval = np.sum(w * np.mean(xf ** 2, axis=(0, 1)))
+    extra = np.sum(w * np.sum(np.abs(m[1:, 0, :]) ** 2, axis=0))
+    assert o.inner_prod(m, m, O.NEUMANN) == pytest.approx(parseval + extra, rel=1e-12)
+
+
+# ---------------------------------------------------------------- time stepping
+def _kh_state(o, seed=0, amp=1e-2):
+    z, zf, DY, DYF = o.grid()
+    U = np.tanh(zf)
+    U[1] = U[2]; U[0] = U[1]; U[-2] = U[-3]; U[-1] = U[-2]
+    o.set_background(U)
+    rng = np.random.default_rng(seed)
+    for name in ("u1", "u3", "b"):
+        zz = z if name == "u3" else zf
+        a = np.zeros(o.modal_shape, complex)
+        a[0, 1:3, :] = amp * (rng.standard_normal((2, 1)) + 1j * rng.standard_normal((2, 1))) * np.exp(-(zz / 2) ** 2)
+        o.set_modal(name, a)
+    o.set_modal("u2", np.zeros(o.modal_shape, complex))
+    o.set_modal("p", np.zeros(o.modal_shape, complex))
+    o.filter_all()
+    o.populate_nodal()
+    o.remove_divergence(0.0)
+
+
+def test_temporal_order():
+    """From a state that already satisfies the discrete boundary rows, u and b
+    converge at second order or better under dt halving, while p converges at
+    first order: RemoveDivergence adds q/h_k to p, which the reference itself
+    calls "effectively forward euler" (IMEXRK.cpp:69-72)."""
+    o = make(N1=24, N3=48, Re=500.0)
+    _kh_state(o)
+    o.set_dt(0.0005)
+    o.step(2)
+    start = {n: o.get_modal(n) for n in ("u1", "u2", "u3", "b", "p")}
+    U = None
+    res = []
+    for dt in (0.04, 0.02, 0.01, 0.005):
+        q = make(N1=24, N3=48, Re=500.0)
+        z, zf, DY, DYF = q.grid()
+        U = np.tanh(zf)
+        U[1] = U[2]; U[0] = U[1]; U[-2] = U[-3]; U[-1] = U[-2]
+        q.set_background(U)
+        for n, v in start.items():
+            q.set_modal(n, v)
+        q.populate_nodal()
+        q.set_dt(dt)
+        q.step(int(round(0.8 / dt)))
+        res.append({n: q.get_modal(n) for n in ("u1", "u3", "b", "p")})
+    for n in ("u1", "u3", "b", "p"):
+        r = [np.linalg.norm(res[k][n] - res[k + 1][n]) / np.linalg.norm(res[k + 1][n] - res[k + 2][n]) for k in (0, 1)]
+        if n == "p":
+            assert all(1.8 < x < 2.3 for x in r), (n, r)
+        else:
+            assert all(x > 3.5 for x in r), (n, r)
+
+
+def test_inconsistent_start_costs_one_order():
+    """Starting from fields that violate the discrete end rows costs O(dt) once."""
+    res = []
+    for dt in (0.04, 0.02, 0.01):
+        o = make(N1=24, N3=48, Re=500.0)
+        _kh_state(o)
+        o.set_dt(dt)
+        o.step(int(round(0.8 / dt)))
+        res.append(o.get_modal("u1"))
+    ratio = np.linalg.norm(res[0] - res[1]) / np.linalg.norm(res[1] - res[2])
+    assert 1.8 < ratio < 2.3, ratio
+
+
+def test_kelvin_helmholtz_growth_rate():
+    """Michalke (1964): tanh shear layer, alpha = 0.4446, sigma ~ 0.19 inviscid; Re = 1000 slightly lower."""
+    o = make(N1=16, N3=128, Re=1000.0, Ri=0.0)
+    _kh_state(o, amp=1e-6)
+    o.set_dt(0.02)
+    E = []
+    for _ in range(16):
+        o.step(100)
+        E.append(o.energy()[0])
+    E = np.array(E)
+    sigma = 0.5 * np.log(E[-1] / E[-2]) / 2.0
+    assert 0.17 < sigma < 0.195, sigma
+
+
+def test_stratification_reduces_growth():
+    rates = []
+    for Ri in (0.0, 0.1):
+        o = make(N1=16, N3=96, Re=1000.0, Ri=Ri)
+        _kh_state(o, amp=1e-6)
+        o.set_dt(0.02)
+        E = []
+        for _ in range(12):
+            o.step(100)
+            E.append(o.energy()[0] + o.energy()[1])
+        rates.append(0.5 * np.log(E[-1] / E[-2]) / 2.0)
+    assert rates[1] < rates[0]
+
+
+def test_fp32_oracle_tracks_fp64_oracle():
+    out = []
+    for prec in (8, 4):
+        o = make(N1=48, N3=64, prec=prec)
+        _kh_state(o, amp=5e-2)
+        o.set_dt(0.01)
+        o.step(50)
+        out.append(o.energy())
+    assert abs(out[1][0] - out[0][0]) / out[0][0] < 1e-4
+    assert abs(out[1][1] - out[0][1]) / out[0][1] < 1e-4
