@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Throughput of the IMEX-RK3/CN timestep (IMEXRK::TimeStep, IMEXRK.cpp:8-24) on one MI355X.
+
+Workload: 3D stratified shear layer, 384 (x, Fourier) x 192 (y, Fourier) x 384
+(z, stretched finite differences), FP32, Lx = 2*pi/0.44, Ly = Lx/2, L3 = 10,
+Re = 4000, Pr = 1, Ri = 0.16, U = tanh(z), B = tanh(z), seeded perturbation of
+amplitude 1e-3, dt = 0.005.  One "step" is one full three-stage TimeStep.
+
+Prints one JSON line.  ``value`` is steps/s with the state resident in HBM,
+timed with HIP events on the solver's stream.  ``roofline`` is the dominant
+kernel's algorithmic bytes per launch over its average launch time;
+``step_roofline`` applies the per-step byte model to the whole step;
+``cpu_baseline`` times the CPU oracle (a C++ restatement of the reference,
+OpenMP) on the host cores for a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "steps/s @384×192×384 fp32; Newton-GMRES time, learned vs plain @1e-8 res"
+GRID = dict(N1=384, N2=192, N3=384)
+FLOW = dict(Re=4000.0, Ri=0.16, Pr=1.0, L3=10.0)
+DT = 0.005
+SEED = 20261202
+
+
+def sizes(N1, N2, N3, word=4):
+    A1 = N1 // 3
+    A2 = 2 * (N2 // 3) - 1 if N2 > 1 else 1
+    cplx = 2 * word
+    Ba = A1 * A2 * N3 * cplx          # one active modal field
+    Bi = N2 * A1 * N3 * cplx          # one x-spectral field (all n2, active j1)
+    Bn = N1 * N2 * N3 * word          # one nodal field
+    return A1, A2, Ba, Bi, Bn
+
+
+def kernel_bytes(N1, N2, N3, word=4):
+    """Algorithmic (compulsory) HBM bytes per launch of each kernel class, 3D."""
+    _, _, Ba, Bi, _ = sizes(N1, N2, N3, word)
+    # modal: per stage reads nl(4)+u(4)+p, r_old(4) on stages 1,2; writes u(4)+p, r_new(4) on stages 0,1
+    modal_avg = ((9 + 13 + 13) + (9 + 9 + 5)) / 3.0 * Ba
+    return {"yinv": 4 * Ba + 4 * Bi, "xpass": 4 * Bi + 8 * Bi, "yfwd": 8 * Bi + 4 * Ba, "modal": modal_avg}
+
+
+def model_step_bytes(N1, N2, N3, word=4):
+    """SF-IO-v1 byte model of one step: 3*[Bn(2nF+2nP) + Ba(5nF+2nP+2)]."""
+    _, _, Ba, _, Bn = sizes(N1, N2, N3, word)
+    nF, nP = (4, 11) if N2 > 1 else (3, 6)
+    return 3 * (Bn * (2 * nF + 2 * nP) + Ba * (5 * nF + 2 * nP + 2))
+
+
+def setup(sf, ic, prec=4):
+    import numpy as np
+    N1, N2, N3 = GRID["N1"], GRID["N2"], GRID["N3"]
+    L1 = 2 * np.pi / 0.44
+    s = sf.IMEXRK(N1, N2, N3, sf.THREE_D, L1, L1 / 2, FLOW["L3"], FLOW["Re"], FLOW["Ri"], FLOW["Pr"], precision=prec)
+    z, zf = s.vertical_points()
+    U, S = ic.tanh_background(zf), ic.tanh_stratification(zf)
+    pert = ic.perturbation(N1, N2, N3, sf.THREE_D, z, zf, SEED, amplitude=1e-3)
+    return s, U, S, pert
+
+
+def cpu_baseline(U, S, pert, max_seconds):
+    """Time the CPU oracle (FP32, OpenMP) on one step of the same workload."""
+    import numpy as np
+    import oracle
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    ncores = max(1, min(ncores, 16))
+    oracle.set_threads(ncores)
+    N1, N2, N3 = GRID["N1"], GRID["N2"], GRID["N3"]
+    L1 = 2 * np.pi / 0.44
+    o = oracle.Oracle(N1, N2, N3, oracle.THREE_D, L1, L1 / 2, FLOW["L3"], FLOW["Re"], FLOW["Ri"], FLOW["Pr"],
+                      precision=4)
+    o.set_background(U, S)
+    for n, a in pert.items():
+        o.set_modal(n, a)
+    o.filter_all()
+    o.populate_nodal()
+    o.remove_divergence(0.0)
+    o.set_dt(DT)
+    t0 = time.perf_counter()
+    o.stage(0)
+    t_stage = time.perf_counter() - t0
+    nstages = 1
+    # complete the step if it fits the budget
+    if 2 * t_stage < max_seconds:
+        t0 = time.perf_counter()
+        o.stage(1)
+        o.stage(2)
+        t_stage = (t_stage + time.perf_counter() - t0) / 3.0
+        nstages = 3
+    step_s = 3.0 * t_stage
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": ncores, "kind": "port",
+            "sample": "%d RK stage(s) of one 384x192x384 FP32 step of the CPU oracle (C++14/OpenMP restatement, own "
+                      "FFT, banded vertical operators), setup excluded; %s" % (nstages, cpu)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=40.0, help="budget for the CPU baseline sample")
+    ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_c2.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the IMEX-RK step is not sharded across GPUs in this build "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import stratiflow_amd as sf
+    from stratiflow_amd import ic
+
+    s, U, S, pert = setup(sf, ic)
+    s.SetBackground(U, S)
+    for n, a in pert.items():
+        s.set(n, a)
+    s.FilterAll()
+    s.PopulateNodalVariables()
+    s.RemoveDivergence(0.0)
+    s.deltaT = DT
+    s.TimeStep(args.warmup)
+    s.synchronize()
+
+    t0 = time.perf_counter()
+    total_ms, per = s.time_steps_detailed(args.steps)
+    s.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    ms_step = total_ms / args.steps
+    ke, pe = s.energies()
+    if not (np.isfinite(ke) and np.isfinite(pe)):
+        raise RuntimeError("non-finite energy after the timed steps")
+
+    N1, N2, N3 = GRID["N1"], GRID["N2"], GRID["N3"]
+    kb = kernel_bytes(N1, N2, N3)
+    dom = max(per, key=lambda k: per[k][0])
+    dom_ms, dom_n = per[dom]
+    achieved = kb[dom] / (dom_ms / dom_n * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic_json):
+        try:
+            traffic = json.load(open(args.traffic_json)).get(dom)
+        except (OSError, ValueError):
+            traffic = None
+    step_bytes = model_step_bytes(N1, N2, N3)
+    step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": 1e3 / ms_step,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "IMEX-RK3/CN TimeStep, 3D stratified shear layer 384x192x384 (x,y Fourier; z "
+                               "stretched FD), Re=4000 Ri=0.16 Pr=1, tanh U and B, dt=0.005, seeded 1e-3 "
+                               "perturbation", "N1": N1, "N2": N2, "N3": N3, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": kb[dom], "ms_per_launch": dom_ms / dom_n},
+        "step_roofline": {"model": "SF-IO-v1", "bytes_per_step": step_bytes, "achieved": step_gbs,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in per.items()},
+        "wall_ms_per_step": wall_ms / args.steps,
+        "energy": {"KE": ke, "PE": pe},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(U, S, pert, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -167,6 +167,14 @@ int sf_synchronize(sf_ctx* ctx);
 /* nsteps of sf_step bracketed by HIP events on the context's stream;
  * *ms = elapsed milliseconds */
 int sf_time_steps(sf_ctx* ctx, int nsteps, double* ms);
+/* as sf_time_steps, with an event after every kernel launch: kernel_ms[c]
+ * and launches[c] accumulate per kernel class c (SF_K_* below) */
+#define SF_K_YINV 0
+#define SF_K_XPASS 1
+#define SF_K_YFWD 2
+#define SF_K_MODAL 3
+#define SF_K_COUNT 4
+int sf_time_steps_detailed(sf_ctx* ctx, int nsteps, double* total_ms, double* kernel_ms, int* launches);
 
 #ifdef __cplusplus
 }

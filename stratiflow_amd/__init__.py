@@ -56,8 +56,9 @@ EXPORTED = [
     "sf_field_zero_ends", "sf_field_neumann_ends", "sf_field_filter", "sf_to_modal", "sf_to_nodal",
     "sf_ddx", "sf_ddy", "sf_ddz", "sf_inner_prod", "sf_state_field", "sf_step", "sf_step_linear",
     "sf_set_linear_background", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
-    "sf_energy", "sf_cfl", "sf_synchronize", "sf_time_steps",
+    "sf_energy", "sf_cfl", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
 ]
+KERNEL_CLASSES = ["yinv", "xpass", "yfwd", "modal"]
 
 
 def lib():
@@ -109,6 +110,7 @@ def lib():
         "sf_cfl": [vp, dp],
         "sf_synchronize": [vp],
         "sf_time_steps": [vp, i, dp],
+        "sf_time_steps_detailed": [vp, i, dp, dp, ctypes.POINTER(ctypes.c_int)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -339,3 +341,11 @@ class IMEXRK:
         ms = ctypes.c_double()
         self._chk(lib().sf_time_steps(self.h, int(n), ctypes.byref(ms)))
         return ms.value
+
+    def time_steps_detailed(self, n):
+        """(total_ms, {class: (ms, launches)}) with one HIP event after every kernel."""
+        ms = ctypes.c_double()
+        k = (ctypes.c_double * len(KERNEL_CLASSES))()
+        c = (ctypes.c_int * len(KERNEL_CLASSES))()
+        self._chk(lib().sf_time_steps_detailed(self.h, int(n), ctypes.byref(ms), k, c))
+        return ms.value, {name: (k[i], c[i]) for i, name in enumerate(KERNEL_CLASSES)}

@@ -124,7 +124,32 @@ struct Ctx {
     V* Y[4] = {nullptr};
     V* Yb[4] = {nullptr};
     V* X[8] = {nullptr};
+    // Spectra the nodal fields were last populated from.  The reference's
+    // products read the nodal fields U1..B, which PopulateNodalVariables
+    // refreshes at the end of every stage but which RemoveDivergence does not
+    // touch (StateVector.cpp:9-11 populates before projecting).  After an
+    // explicit sf_populate_nodal the next stage reads these copies; every
+    // stage ends with nodal fields consistent with the state again.
+    V* snap[4] = {nullptr};
+    bool use_snap = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // per-launch events for sf_time_steps_detailed
+    bool prof = false;
+    std::vector<hipEvent_t> pev;
+    std::vector<int> pcls;
+    size_t pn = 0;
+    void mark(int cls) {
+        if (!prof) return;
+        if (pn >= pev.size()) {
+            hipEvent_t e;
+            SF_HIP(hipEventCreate(&e));
+            pev.push_back(e);
+            pcls.push_back(-1);
+        }
+        SF_HIP(hipEventRecord(pev[pn], stream));
+        pcls[pn] = cls;
+        ++pn;
+    }
 
     template <typename P>
     P* dalloc(size_t count) {
@@ -142,6 +167,7 @@ struct Ctx {
         if (scal_host) (void)hipHostFree(scal_host);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : pev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -331,6 +357,8 @@ struct Ctx {
         if (e != hipSuccess) throw Error(SF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
     }
 
+    const V* src(int i) { return use_snap ? snap[i] : mf(SF_U1 + i); }
+
     // nonlinear (or linearised) explicit terms of the current state into nl[]
     void transform_path(bool linear) {
         if (three) {
@@ -339,14 +367,12 @@ struct Ctx {
             a.A1 = A1;
             a.A2 = A2;
             a.nf = 4;
-            a.in[0] = mf(SF_U1);
-            a.in[1] = mf(SF_U2);
-            a.in[2] = mf(SF_U3);
-            a.in[3] = mf(SF_B);
+            for (int i = 0; i < 4; ++i) a.in[i] = src(i);
             for (int i = 0; i < 4; ++i) a.out[i] = Y[i];
             a.tw = tw2;
             a.scale = T(1);
             check(launch_yinv<T>(a, N2, stream), "yinv");
+            mark(SF_K_YINV);
             if (linear) {
                 if (!Yb[0])
                     for (int i = 0; i < 4; ++i) Yb[i] = dalloc<V>(size_t(N2) * A1 * N3);
@@ -356,6 +382,7 @@ struct Ctx {
                 a.in[3] = mf(SF_B_TOT);
                 for (int i = 0; i < 4; ++i) a.out[i] = Yb[i];
                 check(launch_yinv<T>(a, N2, stream), "yinv(base)");
+                mark(SF_K_YINV);
             }
             XArgs<T> x{};
             x.N3 = N3;
@@ -372,6 +399,7 @@ struct Ctx {
             x.linear = linear ? 1 : 0;
             for (int i = 0; i < 4; ++i) x.base[i] = linear ? Yb[i] : nullptr;
             check(launch_xpass<T>(x, N1, N2, true, stream), "xpass");
+            mark(SF_K_XPASS);
             YArgs<T> f{};
             f.N3 = N3;
             f.A1 = A1;
@@ -387,6 +415,7 @@ struct Ctx {
             f.tw = tw2;
             f.scale = T(1.0 / (double(N1) * double(N2)));
             check(launch_yfwd<T>(f, N2, stream), "yfwd");
+            mark(SF_K_YFWD);
         } else {
             XArgs<T> x{};
             x.N3 = N3;
@@ -394,10 +423,10 @@ struct Ctx {
             x.A1 = A1;
             x.in_n2_stride = 0;
             x.out_n2_stride = 0;
-            x.in[0] = mf(SF_U1);
+            x.in[0] = src(0);
             x.in[1] = nullptr;
-            x.in[2] = mf(SF_U3);
-            x.in[3] = mf(SF_B);
+            x.in[2] = src(2);
+            x.in[3] = src(3);
             x.out[0] = nl[0];
             x.out[1] = nl[1];
             x.out[2] = nl[2];
@@ -413,6 +442,7 @@ struct Ctx {
             x.base[2] = mf(SF_U3_TOT);
             x.base[3] = mf(SF_B_TOT);
             check(launch_xpass<T>(x, N1, 1, false, stream), "xpass");
+            mark(SF_K_XPASS);
         }
     }
 
@@ -466,6 +496,8 @@ struct Ctx {
         m.pmult = T(1) / h[k];
         m.mode = 0;
         check(launch_modal<T>(m, stream), "modal");
+        mark(SF_K_MODAL);
+        use_snap = false;
         if (store_r) {
             rcur = nxt;
             bind_r();
@@ -545,6 +577,12 @@ struct Ctx {
     }
 
     void populate_nodal() {
+        for (int i = 0; i < 4; ++i) {
+            if (i == 1 && !three) continue;
+            if (!snap[i]) snap[i] = dalloc<V>(nmodal);
+            SF_HIP(hipMemcpyAsync(snap[i], mf(SF_U1 + i), nmodal * sizeof(V), hipMemcpyDeviceToDevice, stream));
+        }
+        use_snap = true;
         to_nodal(mf(SF_U1), static_cast<T*>(state[SF_NODAL_U1].d));
         if (three) to_nodal(mf(SF_U2), static_cast<T*>(state[SF_NODAL_U2].d));
         to_nodal(mf(SF_U3), static_cast<T*>(state[SF_NODAL_U3].d));
@@ -1024,6 +1062,43 @@ int sf_time_steps(sf_ctx* ctx, int nsteps, double* ms) {
             float t = 0;
             SF_HIP(hipEventElapsedTime(&t, C.ev0, C.ev1));
             *ms = t;
+        });
+    });
+}
+
+int sf_time_steps_detailed(sf_ctx* ctx, int nsteps, double* total_ms, double* kernel_ms, int* launches) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(total_ms);
+    SF_NULLCHK(kernel_ms);
+    SF_NULLCHK(launches);
+    return guarded(ctx, [&]() {
+        with_ctx(ctx, [&](auto& C) {
+            for (int c = 0; c < SF_K_COUNT; ++c) {
+                kernel_ms[c] = 0;
+                launches[c] = 0;
+            }
+            C.prof = true;
+            C.pn = 0;
+            try {
+                C.mark(-1);
+                C.step(nsteps, false);
+            } catch (...) {
+                C.prof = false;
+                throw;
+            }
+            C.prof = false;
+            SF_HIP(hipEventSynchronize(C.pev[C.pn - 1]));
+            float t = 0;
+            SF_HIP(hipEventElapsedTime(&t, C.pev[0], C.pev[C.pn - 1]));
+            *total_ms = t;
+            for (size_t i = 1; i < C.pn; ++i) {
+                SF_HIP(hipEventElapsedTime(&t, C.pev[i - 1], C.pev[i]));
+                const int c = C.pcls[i];
+                if (c >= 0 && c < SF_K_COUNT) {
+                    kernel_ms[c] += t;
+                    launches[c] += 1;
+                }
+            }
         });
     });
 }

@@ -24,6 +24,7 @@ def make_pair(N1, N2, N3, dim, Re=1000.0, Ri=0.1, Pr=1.0, dt=0.01, seed=1, gpu_p
         o.set_modal(name, arr)
     if project:
         s.FilterAll()
+        s.PopulateNodalVariables()
         s.RemoveDivergence(0.0)
         o.filter_all()
         o.populate_nodal()

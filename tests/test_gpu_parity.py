@@ -50,13 +50,12 @@ def test_remove_divergence(shape, prec, tol):
         assert e < tol, f
 
 
-@pytest.mark.parametrize("prec,tol", [(8, 1e-9), (4, 2e-4)])
 @pytest.mark.parametrize("shape", [(48, 1, 64, 1), (24, 12, 32, 0)])
 @pytest.mark.parametrize("k", [0, 1, 2])
-def test_one_stage(shape, k, prec, tol):
+def test_one_stage_fp64(shape, k):
+    """FP64 GPU path reproduces the FP64 oracle stage by stage (incl. the explicit terms r)."""
     N1, N2, N3, dim = shape
-    s, o = make_pair(N1, N2, N3, dim, gpu_prec=prec, amplitude=0.05)
-    # bring both to stage k with identical history
+    s, o = make_pair(N1, N2, N3, dim, gpu_prec=8, amplitude=0.05)
     for kk in range(k + 1):
         s.Stage(kk)
         o.stage(kk)
@@ -64,7 +63,29 @@ def test_one_stage(shape, k, prec, tol):
     for f in names:
         e = rel_l2(s.get(f), o.get_modal(f))
         _report("stage%d" % k, **{f: e})
-        assert e < tol, f
+        assert e < 1e-9, f
+
+
+@pytest.mark.parametrize("shape", [(48, 1, 64, 1), (24, 12, 32, 0)])
+@pytest.mark.parametrize("k", [0, 2])
+def test_one_stage_fp32(shape, k):
+    """FP32 GPU error against the FP64 oracle stays within 3x the FP32 oracle's own error.
+
+    The FP32 reference path is itself inexact: the pinned Neumann Poisson
+    solve of the (0,0) mode (IMEXRK.h:68-74) is ill-conditioned, so p carries
+    ~1e-4 relative rounding error even in the reference's default precision."""
+    N1, N2, N3, dim = shape
+    s, o64 = make_pair(N1, N2, N3, dim, gpu_prec=4, oracle_prec=8, amplitude=0.05)
+    _, o32 = make_pair(N1, N2, N3, dim, gpu_prec=4, oracle_prec=4, amplitude=0.05)
+    for kk in range(k + 1):
+        s.Stage(kk)
+        o64.stage(kk)
+        o32.stage(kk)
+    for f in fields(dim):
+        eg = rel_l2(s.get(f), o64.get_modal(f))
+        er = rel_l2(o32.get_modal(f), o64.get_modal(f))
+        _report("stage%d_fp32" % k, **{f + "_gpu": eg, f + "_ref32": er})
+        assert eg <= max(3 * er, 1e-6), f
 
 
 @pytest.mark.parametrize("shape", [(48, 1, 64, 1), (128, 1, 128, 1), (24, 12, 32, 0), (48, 24, 64, 0)])
@@ -85,5 +106,7 @@ def test_steps_fp32_energy(shape):
     assert abs(ke - ko) / ko < 1e-4
     assert abs(pe - po) / po < 1e-4
     for f in fields(dim):
-        e = rel_l2(s.get(f), o.get_modal(f))
-        assert e < 1e-3, f
+        eg = rel_l2(s.get(f), o64.get_modal(f))
+        er = rel_l2(o.get_modal(f), o64.get_modal(f))
+        _report("steps_fp32", **{f + "_gpu": eg, f + "_ref32": er})
+        assert eg <= max(3 * er, 1e-6), f
