@@ -1,0 +1,816 @@
+// stratiflow.h -- C++14 facade with the reference's class and function names
+// (Field.h, Stratiflow.h, IMEXRK.h, StateVector.h, Integration.h, Parameters.h)
+// over the C ABI of libsfhip.so (include/sfhip.h).
+//
+// Differences from the reference that a caller sees:
+//  * gridParams is a runtime global (constexpr in Parameters.h:42-43); set it,
+//    and flowParams, before the first field or solver is constructed.
+//  * Fields live on the GPU.  Element access (operator()) goes through a host
+//    mirror that is fetched on first access and uploaded before the next
+//    device operation; modal fields store only the active (dealiased) modes,
+//    so writing an inactive mode is dropped as ModalField::Filter would.
+//  * Inner products, norms and energies are accumulated in FP64 on the GPU.
+//  * Errors are reported by throwing std::runtime_error (the C ABI returns
+//    status codes; the reference asserts or throws std::string).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../sfhip.h"
+
+#ifdef USE_DOUBLE
+typedef double stratifloat;
+#define SF_FACADE_PRECISION 8
+#else
+typedef float stratifloat;
+#define SF_FACADE_PRECISION 4
+#endif
+
+using complex = std::complex<stratifloat>;
+constexpr stratifloat pi = 3.141592653589793238462;  // Constants.h:63
+constexpr std::complex<stratifloat> i(0, 1);          // Constants.h:64
+
+enum class BoundaryCondition { Dirichlet, Neumann };                        // Constants.h:78-82
+enum class Dimensionality { ThreeDimensional, TwoDimensional, TwoAndAHalf };  // Constants.h:84-89
+
+struct GridParams {  // Parameters.h:6-18
+    int N1;
+    int N2;
+    int N3;
+    Dimensionality dimensionality;
+    bool ThirdDimension() const {
+        return dimensionality == Dimensionality::ThreeDimensional || dimensionality == Dimensionality::TwoAndAHalf;
+    }
+};
+
+struct FlowParams {  // Parameters.h:21-30
+    stratifloat L1, L2, L3, Re, Ri, Pr;
+    bool EvolveBackground;
+};
+
+inline stratifloat InitialU(stratifloat z) { return std::tanh(z); }  // Parameters.h:33-36
+
+namespace sfacade {
+
+template <typename D = void>
+struct Globals {
+    static GridParams grid;
+    static FlowParams flow;
+    static sf_ctx* ctx;
+    static bool ctx_three;
+};
+// defaults of Parameters.h:42-43 and Parameters.cpp:6-7
+template <typename D>
+GridParams Globals<D>::grid = {48, 1, 512, Dimensionality::TwoDimensional};
+template <typename D>
+FlowParams Globals<D>::flow = {stratifloat(8.885765876 / 2), stratifloat(1.570795), 10, 1000, stratifloat(0.16), 7,
+                               false};
+template <typename D>
+sf_ctx* Globals<D>::ctx = nullptr;
+template <typename D>
+bool Globals<D>::ctx_three = false;
+
+inline void check(int rc, const sf_ctx* c) {
+    if (rc != SF_OK) throw std::runtime_error(std::string("sfhip: ") + sf_last_error(c));
+}
+
+inline int dim_code(Dimensionality d) {
+    return d == Dimensionality::ThreeDimensional ? SF_THREE_D
+                                                 : (d == Dimensionality::TwoDimensional ? SF_TWO_D : SF_TWO_AND_A_HALF);
+}
+inline int bc_code(BoundaryCondition b) { return b == BoundaryCondition::Neumann ? SF_NEUMANN : SF_DIRICHLET; }
+
+inline sf_flow flow_c() {
+    const FlowParams& f = Globals<>::flow;
+    sf_flow c;
+    c.L1 = f.L1;
+    c.L2 = f.L2;
+    c.L3 = f.L3;
+    c.Re = f.Re;
+    c.Ri = f.Ri;
+    c.Pr = f.Pr;
+    c.evolve_background = f.EvolveBackground ? 1 : 0;
+    return c;
+}
+
+// the process-wide solver context (the reference's StateVector::solver
+// singleton, StateVector.cpp:238)
+inline sf_ctx* ctx() {
+    if (!Globals<>::ctx) {
+        const GridParams& g = Globals<>::grid;
+        sf_grid gc = {g.N1, g.N2, g.N3, dim_code(g.dimensionality)};
+        sf_flow fc = flow_c();
+        sf_ctx* c = nullptr;
+        check(sf_ctx_create(&gc, &fc, SF_FACADE_PRECISION, &c), nullptr);
+        Globals<>::ctx = c;
+        Globals<>::ctx_three = g.ThirdDimension();
+    }
+    return Globals<>::ctx;
+}
+
+}  // namespace sfacade
+
+static GridParams& gridParams = sfacade::Globals<>::grid;
+static FlowParams& flowParams = sfacade::Globals<>::flow;
+
+// ---------------------------------------------------------------------------
+// Fields
+// ---------------------------------------------------------------------------
+class DeviceField {
+public:
+    DeviceField(int kind, BoundaryCondition bc) : kind_(kind), bc_(bc) {
+        sfacade::check(sf_field_alloc(sfacade::ctx(), kind, sfacade::bc_code(bc), &f_), sfacade::ctx());
+        owned_ = true;
+    }
+    // borrow a solver-owned field
+    DeviceField(sf_field* f, int kind, BoundaryCondition bc) : f_(f), kind_(kind), bc_(bc), owned_(false) {}
+    DeviceField(const DeviceField& o) : DeviceField(o.kind_, o.bc_) { copy_from(o); }
+    DeviceField& operator=(const DeviceField& o) {
+        copy_from(o);
+        return *this;
+    }
+    virtual ~DeviceField() {
+        if (owned_ && f_) sf_field_free(f_);
+    }
+
+    BoundaryCondition BC() const { return bc_; }
+
+    // device handle after pending host writes are uploaded; 'write' drops the mirror
+    sf_field* dev(bool write) const {
+        if (dirty_) {
+            upload();
+            dirty_ = false;
+        }
+        if (write) valid_ = false;
+        return f_;
+    }
+
+    void Zero() { sfacade::check(sf_field_zero(dev(true)), sfacade::ctx()); }
+
+    DeviceField& operator*=(stratifloat a) {
+        sfacade::check(sf_field_scale(dev(true), a), sfacade::ctx());
+        return *this;
+    }
+    DeviceField& operator+=(const DeviceField& o) {
+        axpby(1, o, 1);
+        return *this;
+    }
+    DeviceField& operator-=(const DeviceField& o) {
+        axpby(-1, o, 1);
+        return *this;
+    }
+    // this = a*x + b*this
+    void axpby(stratifloat a, const DeviceField& x, stratifloat b) {
+        sfacade::check(sf_field_axpby(dev(true), a, x.dev(false), b), sfacade::ctx());
+    }
+
+protected:
+    void copy_from(const DeviceField& o) {
+        if (&o == this) return;
+        if (o.bc_ != bc_) throw std::runtime_error("assignment between fields of different boundary conditions");
+        sfacade::check(sf_field_copy(dev(true), o.dev(false)), sfacade::ctx());
+    }
+    virtual size_t host_doubles() const = 0;
+    void fetch() const {
+        if (valid_) return;
+        hostd_.resize(host_doubles());
+        sfacade::check(sf_field_download(f_, hostd_.data()), sfacade::ctx());
+        valid_ = true;
+    }
+    void upload() const { sfacade::check(sf_field_upload(f_, hostd_.data()), sfacade::ctx()); }
+    double* host_write() {
+        if (dirty_ == false) fetch();
+        dirty_ = true;
+        return hostd_.data();
+    }
+    const double* host_read() const {
+        fetch();
+        return hostd_.data();
+    }
+
+    sf_field* f_ = nullptr;
+    int kind_;
+    BoundaryCondition bc_;
+    bool owned_ = false;
+    mutable std::vector<double> hostd_;
+    mutable bool valid_ = false;
+    mutable bool dirty_ = false;
+};
+
+// element proxies (the reference returns T&; the facade returns a proxy that
+// writes through the host mirror)
+class ModalRef {
+public:
+    ModalRef(double* p) : p_(p) {}
+    operator complex() const { return complex(stratifloat(p_[0]), stratifloat(p_[1])); }
+    ModalRef& operator=(complex v) {
+        p_[0] = v.real();
+        p_[1] = v.imag();
+        return *this;
+    }
+    ModalRef& operator+=(complex v) { return *this = complex(*this) + v; }
+    complex value() const { return *this; }
+
+private:
+    double* p_;
+};
+
+class NodalField;
+
+class ModalField : public DeviceField {  // Field.h:925-1207
+public:
+    ModalField(BoundaryCondition bc) : DeviceField(SF_MODAL, bc) {}
+    ModalField(sf_field* f, BoundaryCondition bc) : DeviceField(f, SF_MODAL, bc) {}
+    ModalField(const ModalField&) = default;
+    ModalField& operator=(const ModalField&) = default;
+    using DeviceField::operator=;
+
+    ModalRef operator()(int j1, int j2, int j3) { return ModalRef(host_write() + 2 * index(j1, j2, j3)); }
+    complex operator()(int j1, int j2, int j3) const {
+        const double* p = host_read() + 2 * index(j1, j2, j3);
+        return complex(stratifloat(p[0]), stratifloat(p[1]));
+    }
+
+    void ToNodal(NodalField& other) const;
+    void ZeroEnds() { sfacade::check(sf_field_zero_ends(dev(true)), sfacade::ctx()); }
+    void NeumannEnds() { sfacade::check(sf_field_neumann_ends(dev(true)), sfacade::ctx()); }
+    void Filter() { sfacade::check(sf_field_filter(dev(true)), sfacade::ctx()); }
+
+    // ModalField::PhaseShift (Field.h:1168-1174), on the host mirror
+    void PhaseShift(stratifloat shift) {
+        const GridParams& g = gridParams;
+        double* p = host_write();
+        for (int j2 = 0; j2 < g.N2; ++j2)
+            for (int j1 = 0; j1 < g.N1 / 3; ++j1) {
+                const std::complex<double> f = std::exp(std::complex<double>(0, j1 * double(shift)));
+                for (int j3 = 0; j3 < g.N3; ++j3) {
+                    double* q = p + 2 * index(j1, j2, j3);
+                    const std::complex<double> v = std::complex<double>(q[0], q[1]) * f;
+                    q[0] = v.real();
+                    q[1] = v.imag();
+                }
+            }
+    }
+
+protected:
+    size_t host_doubles() const override {
+        const GridParams& g = gridParams;
+        return size_t(2) * (g.N1 / 2 + 1) * g.N2 * g.N3;
+    }
+    static size_t index(int j1, int j2, int j3) {
+        const GridParams& g = gridParams;
+        return (size_t(g.N1 / 2 + 1) * j2 + j1) * g.N3 + j3;  // Field.h:399-410
+    }
+};
+
+class NodalField : public DeviceField {  // Field.h:699-923
+public:
+    NodalField(BoundaryCondition bc) : DeviceField(SF_NODAL, bc) {}
+    NodalField(sf_field* f, BoundaryCondition bc) : DeviceField(f, SF_NODAL, bc) {}
+    NodalField(const NodalField&) = default;
+    NodalField& operator=(const NodalField&) = default;
+    using DeviceField::operator=;
+
+    double& operator()(int n1, int n2, int n3) { return host_write()[index(n1, n2, n3)]; }
+    stratifloat operator()(int n1, int n2, int n3) const { return stratifloat(host_read()[index(n1, n2, n3)]); }
+
+    void ToModal(ModalField& other) const {
+        sfacade::check(sf_to_modal(dev(false), other.dev(true)), sfacade::ctx());
+    }
+    stratifloat Max() const {  // Field.h:771-786
+        const double* p = host_read();
+        double m = 0;
+        for (size_t k = 0; k < host_doubles(); ++k) m = std::max(m, std::fabs(p[k]));
+        return stratifloat(m);
+    }
+    void Save(std::ofstream& s) const {  // Field.h:470-473: raw stratifloat values
+        const double* p = host_read();
+        std::vector<stratifloat> v(p, p + host_doubles());
+        s.write(reinterpret_cast<const char*>(v.data()), sizeof(stratifloat) * v.size());
+    }
+    void Load(std::ifstream& s, bool twoDimensional = false) {  // Field.h:903-920
+        const GridParams& g = gridParams;
+        std::vector<stratifloat> v(host_doubles());
+        if (twoDimensional) {
+            const size_t plane = size_t(g.N1) * g.N3;
+            s.read(reinterpret_cast<char*>(v.data()), sizeof(stratifloat) * plane);
+            for (int n = 1; n < g.N2; ++n) std::memcpy(&v[n * plane], &v[(n - 1) * plane], sizeof(stratifloat) * plane);
+        } else {
+            s.read(reinterpret_cast<char*>(v.data()), sizeof(stratifloat) * v.size());
+        }
+        double* p = host_write();
+        for (size_t k = 0; k < v.size(); ++k) p[k] = v[k];
+    }
+
+protected:
+    size_t host_doubles() const override {
+        const GridParams& g = gridParams;
+        return size_t(g.N1) * g.N2 * g.N3;
+    }
+    static size_t index(int n1, int n2, int n3) {
+        const GridParams& g = gridParams;
+        return (size_t(g.N1) * n2 + n1) * g.N3 + n3;
+    }
+};
+
+inline void ModalField::ToNodal(NodalField& other) const {
+    sfacade::check(sf_to_nodal(dev(false), other.dev(true)), sfacade::ctx());
+}
+
+// Stratiflow.h:8-35
+class NeumannModal : public ModalField {
+public:
+    NeumannModal() : ModalField(BoundaryCondition::Neumann) {}
+    NeumannModal(sf_field* f) : ModalField(f, BoundaryCondition::Neumann) {}
+    NeumannModal(const NeumannModal&) = default;
+    NeumannModal& operator=(const NeumannModal&) = default;
+    using ModalField::operator=;
+};
+class DirichletModal : public ModalField {
+public:
+    DirichletModal() : ModalField(BoundaryCondition::Dirichlet) {}
+    DirichletModal(sf_field* f) : ModalField(f, BoundaryCondition::Dirichlet) {}
+    DirichletModal(const DirichletModal&) = default;
+    DirichletModal& operator=(const DirichletModal&) = default;
+    using ModalField::operator=;
+};
+class NeumannNodal : public NodalField {
+public:
+    NeumannNodal() : NodalField(BoundaryCondition::Neumann) {}
+    NeumannNodal(sf_field* f) : NodalField(f, BoundaryCondition::Neumann) {}
+    NeumannNodal(const NeumannNodal&) = default;
+    NeumannNodal& operator=(const NeumannNodal&) = default;
+    using NodalField::operator=;
+};
+class DirichletNodal : public NodalField {
+public:
+    DirichletNodal() : NodalField(BoundaryCondition::Dirichlet) {}
+    DirichletNodal(sf_field* f) : NodalField(f, BoundaryCondition::Dirichlet) {}
+    DirichletNodal(const DirichletNodal&) = default;
+    DirichletNodal& operator=(const DirichletNodal&) = default;
+    using NodalField::operator=;
+};
+
+// Neumann1D / Dirichlet1D (Stratiflow.h:37-49) are host arrays here; the
+// solver uploads U_ when it is set.
+class Nodal1D {
+public:
+    Nodal1D(BoundaryCondition bc) : bc_(bc), v_(gridParams.N3, 0) {}
+    BoundaryCondition BC() const { return bc_; }
+    std::vector<stratifloat>& Get() { return v_; }
+    const std::vector<stratifloat>& Get() const { return v_; }
+    // Nodal1D::SetValue (Field.h:655-682)
+    void SetValue(std::function<stratifloat(stratifloat)> f, stratifloat L3);
+    stratifloat Max() const { return *std::max_element(v_.begin(), v_.end()); }
+    stratifloat Min() const { return *std::min_element(v_.begin(), v_.end()); }
+
+private:
+    BoundaryCondition bc_;
+    std::vector<stratifloat> v_;
+};
+class Neumann1D : public Nodal1D {
+public:
+    Neumann1D() : Nodal1D(BoundaryCondition::Neumann) {}
+};
+class Dirichlet1D : public Nodal1D {
+public:
+    Dirichlet1D() : Nodal1D(BoundaryCondition::Dirichlet) {}
+};
+
+// VerticalPoints / VerticalPointsFractional (Field.cpp:3-31) from the library
+inline void VerticalPointsBoth(std::vector<double>& z, std::vector<double>& zf) {
+    z.assign(gridParams.N3, 0.0);
+    zf.assign(gridParams.N3, 0.0);
+    sfacade::check(sf_vertical_points(sfacade::ctx(), z.data(), zf.data()), sfacade::ctx());
+}
+
+inline void Nodal1D::SetValue(std::function<stratifloat(stratifloat)> f, stratifloat L3) {
+    (void)L3;  // the context's L3 is flowParams.L3
+    std::vector<double> z, zf;
+    VerticalPointsBoth(z, zf);
+    const int N = gridParams.N3;
+    for (int j = 0; j < N; ++j) v_[j] = f(stratifloat(zf[j]));
+    if (bc_ == BoundaryCondition::Neumann) {
+        v_[1] = v_[2];
+        v_[0] = v_[1];
+        v_[N - 2] = v_[N - 3];
+        v_[N - 1] = v_[N - 2];
+    } else {
+        v_[1] = -v_[2];
+        v_[0] = v_[1];
+        v_[N - 1] = -v_[N - 2];
+    }
+}
+
+// ddx / ddy / ddz (Stratiflow.h:51-80) evaluated into a new field
+inline NeumannModal ddx(const NeumannModal& f) {
+    NeumannModal r;
+    sfacade::check(sf_ddx(f.dev(false), r.dev(true)), sfacade::ctx());
+    return r;
+}
+inline DirichletModal ddx(const DirichletModal& f) {
+    DirichletModal r;
+    sfacade::check(sf_ddx(f.dev(false), r.dev(true)), sfacade::ctx());
+    return r;
+}
+inline NeumannModal ddy(const NeumannModal& f) {
+    NeumannModal r;
+    sfacade::check(sf_ddy(f.dev(false), r.dev(true)), sfacade::ctx());
+    return r;
+}
+inline DirichletModal ddy(const DirichletModal& f) {
+    DirichletModal r;
+    sfacade::check(sf_ddy(f.dev(false), r.dev(true)), sfacade::ctx());
+    return r;
+}
+inline DirichletModal ddz(const NeumannModal& f) {
+    DirichletModal r;
+    sfacade::check(sf_ddz(f.dev(false), r.dev(true)), sfacade::ctx());
+    return r;
+}
+inline NeumannModal ddz(const DirichletModal& f) {
+    NeumannModal r;
+    sfacade::check(sf_ddz(f.dev(false), r.dev(true)), sfacade::ctx());
+    return r;
+}
+
+// InnerProd (Integration.h:117-146), FP64 on the GPU
+inline stratifloat InnerProd(const ModalField& a, const ModalField& b, stratifloat L3) {
+    (void)L3;
+    double out = 0;
+    sfacade::check(sf_inner_prod(a.dev(false), b.dev(false), &out), sfacade::ctx());
+    return stratifloat(out);
+}
+inline double InnerProdF64(const ModalField& a, const ModalField& b) {
+    double out = 0;
+    sfacade::check(sf_inner_prod(a.dev(false), b.dev(false), &out), sfacade::ctx());
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// IMEXRK (IMEXRK.h:31-670): a view on the context's solver state
+// ---------------------------------------------------------------------------
+class IMEXRK {
+public:
+    stratifloat deltaT = 0.01f;  // IMEXRK.h:34
+
+    IMEXRK()
+        : u1(state(SF_U1)), u2(state(SF_U2)), b(state(SF_B)), p(state(SF_P)), u3(state(SF_U3)),
+          U1(state(SF_NODAL_U1)), U2(state(SF_NODAL_U2)), B(state(SF_NODAL_B)), U3(state(SF_NODAL_U3)) {
+        UpdateForTimestep();
+    }
+
+    void UpdateForTimestep() { sfacade::check(sf_set_dt(c(), deltaT), c()); }  // IMEXRK.h:502-545
+    void TimeStep() {                                                          // IMEXRK.cpp:8-24
+        sync_dt();
+        sfacade::check(sf_step(c(), 1), c());
+    }
+    void TimeStepLinear() {  // IMEXRK.cpp:26-41
+        sync_dt();
+        sfacade::check(sf_step_linear(c(), 1), c());
+    }
+    void RemoveDivergence(stratifloat pressureMultiplier = 1.0) {  // IMEXRK.cpp:43-73
+        flush_state();
+        sfacade::check(sf_remove_divergence(c(), pressureMultiplier), c());
+        invalidate_state();
+    }
+    void FilterAll() { sfacade::check(sf_filter_all(c()), c()); }  // IMEXRK.h:128-139
+    void PopulateNodalVariables() {                                 // IMEXRK.h:141-162
+        flush_state();
+        sfacade::check(sf_populate_nodal(c()), c());
+        U1.dev(true);
+        U2.dev(true);
+        U3.dev(true);
+        B.dev(true);
+    }
+    void SetBackground(const Neumann1D& velocity) {  // IMEXRK.h:321-324
+        U_ = velocity;
+        std::vector<double> u(U_.Get().begin(), U_.Get().end());
+        sfacade::check(sf_set_background(c(), u.data(), nullptr), c());
+    }
+    void SetBackground(std::function<stratifloat(stratifloat)> velocity) {  // IMEXRK.h:326-332
+        Neumann1D Ubar;
+        Ubar.SetValue(velocity, flowParams.L3);
+        SetBackground(Ubar);
+    }
+    // IMEXRK.h:334-352
+    void SetBackground(const NeumannModal& velocity1, const NeumannModal& velocity2, const DirichletModal& velocity3,
+                       const NeumannModal& buoyancy) {
+        sfacade::check(sf_set_linear_background(c(), velocity1.dev(false),
+                                                gridParams.dimensionality == Dimensionality::ThreeDimensional
+                                                    ? velocity2.dev(false)
+                                                    : nullptr,
+                                                velocity3.dev(false), buoyancy.dev(false)),
+                       c());
+    }
+    stratifloat CFL() {  // IMEXRK.h:355-374
+        sync_dt();
+        double cfl = 0;
+        sfacade::check(sf_cfl(c(), &cfl), c());
+        constexpr stratifloat targetCFL = 0.8;
+        deltaT *= targetCFL / stratifloat(cfl);
+        UpdateForTimestep();
+        return stratifloat(cfl);
+    }
+    stratifloat KE() const {
+        double ke = 0, pe = 0;
+        sfacade::check(sf_energy(c(), &ke, &pe), c());
+        return stratifloat(ke);
+    }
+    stratifloat PE() const {
+        double ke = 0, pe = 0;
+        sfacade::check(sf_energy(c(), &ke, &pe), c());
+        return stratifloat(pe);
+    }
+    // .fields files: nodal U1, U2, U3, B (IMEXRK.h:415-443)
+    void SaveFlow(const std::string& filename) {
+        std::ofstream s(filename, std::ios::out | std::ios::binary);
+        U1.Save(s);
+        U2.Save(s);
+        U3.Save(s);
+        B.Save(s);
+    }
+    void LoadFlow(const std::string& filename, bool twoDimensional) {
+        std::ifstream s(filename, std::ios::in | std::ios::binary);
+        U1.Load(s, twoDimensional);
+        U2.Load(s, twoDimensional);
+        U3.Load(s, twoDimensional);
+        B.Load(s, twoDimensional);
+        if (twoDimensional) U2.Zero();
+        U1.ToModal(u1);
+        if (gridParams.ThirdDimension()) U2.ToModal(u2);
+        U3.ToModal(u3);
+        B.ToModal(b);
+    }
+
+    void flush_state() {
+        u1.dev(false);
+        u2.dev(false);
+        u3.dev(false);
+        b.dev(false);
+        p.dev(false);
+    }
+    void invalidate_state() {
+        u1.dev(true);
+        u2.dev(true);
+        u3.dev(true);
+        b.dev(true);
+        p.dev(true);
+    }
+
+    NeumannModal u1, u2, b, p;  // IMEXRK.h:605-606
+    DirichletModal u3;
+    Neumann1D U_;                       // IMEXRK.h:609
+    NeumannNodal U1, U2, B;             // IMEXRK.h:618-619
+    DirichletNodal U3;
+
+private:
+    static sf_ctx* c() { return sfacade::ctx(); }
+    static sf_field* state(int which) {
+        sf_field* f = nullptr;
+        sfacade::check(sf_state_field(sfacade::ctx(), which, &f), sfacade::ctx());
+        return f;
+    }
+    void sync_dt() {
+        double d = 0;
+        sfacade::check(sf_get_dt(c(), &d), c());
+        if (stratifloat(d) != deltaT) UpdateForTimestep();
+        flush_state();
+        invalidate_state();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// StateVector (StateVector.h:8-552)
+// ---------------------------------------------------------------------------
+class StateVector {
+public:
+    StateVector() { Zero(); }
+    StateVector(const StateVector& other) {  // StateVector.h:16-23 (p is not copied)
+        u1 = other.u1;
+        u2 = other.u2;
+        u3 = other.u3;
+        b = other.b;
+        EnforceBCs();
+    }
+    const StateVector& operator=(const StateVector& other) {  // StateVector.h:93-101
+        u1 = other.u1;
+        u2 = other.u2;
+        u3 = other.u3;
+        b = other.b;
+        EnforceBCs();
+        return *this;
+    }
+
+    NeumannModal u1;
+    NeumannModal u2;
+    DirichletModal u3;
+    NeumannModal b;
+    NeumannModal p;
+
+    const StateVector& operator+=(const StateVector& o) { return MulAdd(1, o); }
+    const StateVector& operator-=(const StateVector& o) { return MulAdd(-1, o); }
+    const StateVector& MulAdd(stratifloat a, const StateVector& B) {  // StateVector.h:67-78
+        u1.axpby(a, B.u1, 1);
+        if (gridParams.ThirdDimension()) u2.axpby(a, B.u2, 1);
+        u3.axpby(a, B.u3, 1);
+        b.axpby(a, B.b, 1);
+        EnforceBCs();
+        return *this;
+    }
+    const StateVector& operator*=(stratifloat a) {
+        u1 *= a;
+        if (gridParams.ThirdDimension()) u2 *= a;
+        u3 *= a;
+        b *= a;
+        EnforceBCs();
+        return *this;
+    }
+    // Dot (StateVector.h:122-134), accumulated in FP64
+    double DotF64(const StateVector& o) const {
+        double prod = InnerProdF64(u1, o.u1) + InnerProdF64(u3, o.u3) + flowParams.Ri * InnerProdF64(b, o.b);
+        if (gridParams.ThirdDimension()) prod += InnerProdF64(u2, o.u2);
+        return prod;
+    }
+    stratifloat Dot(const StateVector& o) const { return stratifloat(DotF64(o)); }
+    stratifloat Norm2() const { return Dot(*this); }
+    stratifloat Energy() const { return stratifloat(0.5 * DotF64(*this)); }
+    stratifloat Norm() const { return stratifloat(std::sqrt(DotF64(*this))); }
+
+    void Zero() {
+        u1.Zero();
+        if (gridParams.ThirdDimension()) u2.Zero();
+        u3.Zero();
+        b.Zero();
+        p.Zero();
+    }
+    void EnforceBCs() {  // StateVector.h:415-421
+        u3.ZeroEnds();
+        u1.NeumannEnds();
+        u2.NeumannEnds();
+        b.NeumannEnds();
+    }
+
+    stratifloat RemovePhaseShift(stratifloat shift) {
+        u1.PhaseShift(shift);
+        if (gridParams.ThirdDimension()) u2.PhaseShift(shift);
+        u3.PhaseShift(shift);
+        b.PhaseShift(shift);
+        return shift;
+    }
+    stratifloat RemovePhaseShift() {
+        const stratifloat shift = -std::arg(complex(u1(1, 0, gridParams.N3 / 2))) + pi / 2;
+        return RemovePhaseShift(shift);
+    }
+
+    // StateVector.cpp:85-105
+    void FixedEvolve(stratifloat deltaT, int steps, std::vector<StateVector>& result) const {
+        result.resize(steps);
+        CopyToSolver();
+        solver().FilterAll();
+        solver().PopulateNodalVariables();
+        solver().RemoveDivergence(0.0f);
+        solver().deltaT = deltaT;
+        solver().UpdateForTimestep();
+        for (int step = 0; step < steps; ++step) {
+            CopyFromSolver(result[step]);
+            solver().TimeStep();
+        }
+    }
+
+    // StateVector.cpp:3-83 without plotting or snapshots
+    stratifloat FullEvolve(stratifloat T, StateVector& result, bool snapshot = false, bool screenshot = true,
+                           bool calcmixing = false) const {
+        (void)snapshot;
+        (void)screenshot;
+        (void)calcmixing;
+        CopyToSolver();
+        solver().SetBackground(InitialU);
+        solver().FilterAll();
+        solver().PopulateNodalVariables();
+        solver().RemoveDivergence(0.0f);
+        stratifloat t = 0.0f;
+        int step = 0;
+        const int stepinterval = 100;
+        while (t + 0.0001 < T) {
+            if (step % stepinterval == 0) {
+                solver().CFL();
+                const stratifloat remaining = T - t;
+                const int remainingSteps = int(remaining / solver().deltaT) + 1;
+                if (remainingSteps < stepinterval) {
+                    solver().deltaT = remaining / remainingSteps;
+                    solver().UpdateForTimestep();
+                }
+            }
+            solver().TimeStep();
+            t += solver().deltaT;
+            step++;
+        }
+        CopyFromSolver(result);
+        return 0;
+    }
+
+    // StateVector.cpp:107-156
+    void LinearEvolve(stratifloat T, const StateVector& about, StateVector& result) const {
+        CopyToSolver();
+        solver().SetBackground(InitialU);
+        solver().SetBackground(about.u1, about.u2, about.u3, about.b);
+        solver().FilterAll();
+        solver().PopulateNodalVariables();
+        solver().RemoveDivergence(0.0f);
+        stratifloat t = 0.0f;
+        solver().deltaT = 0.01f;
+        solver().UpdateForTimestep();
+        while (t + 0.0001 < T) {
+            solver().TimeStepLinear();
+            t += solver().deltaT;
+        }
+        CopyFromSolver(result);
+    }
+
+    // StateVector.cpp:189-226
+    void Rescale(stratifloat energy) {
+        CopyToSolver();
+        solver().FilterAll();
+        solver().PopulateNodalVariables();
+        solver().RemoveDivergence(0.0f);
+        CopyFromSolver();
+        const stratifloat before = Energy();
+        const stratifloat scale = before != 0.0f ? std::sqrt(energy / before) : 0.0f;
+        u1 *= scale;
+        if (gridParams.ThirdDimension()) u2 *= scale;
+        u3 *= scale;
+        b *= scale;
+        CopyToSolver();
+        solver().FilterAll();
+        solver().PopulateNodalVariables();
+        solver().RemoveDivergence(0.0f);
+        CopyFromSolver();
+    }
+
+    void SaveToFile(const std::string& filename) const {  // StateVector.h:260-273
+        CopyToSolver();
+        solver().PopulateNodalVariables();
+        const bool ext = filename.size() >= 7 && filename.compare(filename.size() - 7, 7, ".fields") == 0;
+        solver().SaveFlow(ext ? filename : filename + ".fields");
+    }
+    void LoadFromFile(const std::string& filename, bool twoDimensional = false) {  // StateVector.h:245-258
+        solver().LoadFlow(filename, twoDimensional);
+        CopyFromSolver();
+    }
+
+    static IMEXRK& solver() {  // StateVector.h:546
+        static IMEXRK s;
+        return s;
+    }
+    static void ResetForParams() {  // StateVector.h:505-508
+        sf_flow f = sfacade::flow_c();
+        sfacade::check(sf_set_flow(sfacade::ctx(), &f), sfacade::ctx());
+        solver().UpdateForTimestep();
+    }
+
+private:
+    void CopyToSolver() const {  // StateVector.h:511-525
+        solver().u1 = u1;
+        if (gridParams.ThirdDimension()) solver().u2 = u2;
+        else solver().u2.Zero();
+        solver().u3 = u3;
+        solver().b = b;
+        solver().p = p;
+    }
+    void CopyFromSolver() { CopyFromSolver(*this); }
+    void CopyFromSolver(StateVector& into) const {  // StateVector.h:532-543
+        into.u1 = solver().u1;
+        if (gridParams.ThirdDimension()) into.u2 = solver().u2;
+        into.u3 = solver().u3;
+        into.b = solver().b;
+        into.p = solver().p;
+        into.EnforceBCs();
+    }
+};
+
+inline StateVector operator+(const StateVector& l, const StateVector& r) {
+    StateVector ret = l;
+    ret += r;
+    return ret;
+}
+inline StateVector operator-(const StateVector& l, const StateVector& r) {
+    StateVector ret = l;
+    ret -= r;
+    return ret;
+}
+inline StateVector operator*(stratifloat s, const StateVector& v) {
+    StateVector ret = v;
+    ret *= s;
+    return ret;
+}
+inline StateVector operator*(const StateVector& v, stratifloat s) { return s * v; }

@@ -600,6 +600,30 @@ struct Ctx {
         for (int i = 0; i < n; ++i) out[i] = scal_host[i];
     }
 
+    // IMEXRK::CFL (IMEXRK.h:355-374) for the current dt, without changing it
+    double cfl() {
+        // the reference reads whatever U1..U3 hold (IMEXRK.h:359-365): the
+        // populated snapshot if one is pending, else the current state
+        to_nodal(src(0), static_cast<T*>(state[SF_NODAL_U1].d));
+        if (three) to_nodal(src(1), static_cast<T*>(state[SF_NODAL_U2].d));
+        to_nodal(src(2), static_cast<T*>(state[SF_NODAL_U3].d));
+        const VGrid g = vertical_grid(L3, N3);
+        const double d1 = L1 / N1, d2 = L2 / N2, d3 = g.z[N3 / 2 + 1] - g.z[N3 / 2];
+        const T* ubar = zc + size_t(ZC_UBAR) * N3p;
+        check(launch_maxabs<T>(static_cast<T*>(state[SF_NODAL_U1].d), nnodal, N3, ubar, partial, scal, 0, stream),
+              "cfl");
+        if (three)
+            check(launch_maxabs<T>(static_cast<T*>(state[SF_NODAL_U2].d), nnodal, N3, nullptr, partial, scal, 1,
+                                   stream),
+                  "cfl");
+        check(launch_maxabs<T>(static_cast<T*>(state[SF_NODAL_U3].d), nnodal, N3, nullptr, partial, scal, 2, stream),
+              "cfl");
+        double m[3];
+        read_scalars(3, m);
+        if (!three) m[1] = 0;
+        return (m[0] / d1 + m[1] / d2 + m[2] / d3) * dt;
+    }
+
     void energy(double* ke, double* pe) {
         dot_async(mf(SF_U1), mf(SF_U1), SF_NEUMANN, 0);
         dot_async(mf(SF_U3), mf(SF_U3), SF_DIRICHLET, 1);
@@ -1041,8 +1065,7 @@ int sf_energy(sf_ctx* ctx, double* ke, double* pe) {
 int sf_cfl(sf_ctx* ctx, double* cfl) {
     SF_NULLCHK(ctx);
     SF_NULLCHK(cfl);
-    ctx->err = "sf_cfl is not implemented yet";
-    return SF_ERR_UNSUPPORTED;
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { *cfl = C.cfl(); }); });
 }
 
 int sf_synchronize(sf_ctx* ctx) {

@@ -74,6 +74,9 @@ template <typename T>
 hipError_t launch_dot(const cx<T>* a, const cx<T>* b, int nmodes, int N3, const T* w, double* partial,
                       double* out, int slot, hipStream_t s);
 template <typename T>
+hipError_t launch_maxabs(const T* f, size_t n, int N3, const T* add, double* partial, double* out, int slot,
+                         hipStream_t s);
+template <typename T>
 hipError_t launch_axpby(cx<T>* y, T a, const cx<T>* x, T b, size_t n, hipStream_t s);
 template <typename T>
 hipError_t launch_ends(cx<T>* f, int nstacks, int N3, int bc_op, hipStream_t s);

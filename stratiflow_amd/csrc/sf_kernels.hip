@@ -20,6 +20,8 @@
 #include "sf_kernels.h"
 #include "sf_tri.h"
 
+#include <type_traits>
+
 namespace sf {
 
 constexpr int BS = 256;
@@ -275,19 +277,54 @@ __global__ __launch_bounds__(BS) void k_modal(ModalArgs<T> A) {
             dv[t] = cadd(h, dz);
         }
         zero_ends(dv, false);
-        TriFactor<T, M> F;
-        T ta[M], tb[M], tc[M];
-        build_rows<T, M>(3, A, row, kk, kl, zero_mode, T(0), ta, tb, tc);
-        tri_factor<T, M>(F, ta, tb, tc, lane);
-        tri_apply<T, M>(F, dv, lane);  // dv now holds q
-        V qm[M];
-        nb_prev<T, M>(dv, qm, lane);
+        if (std::is_same<T, float>::value && zero_mode) {
+            // The pinned Neumann Poisson problem of the (0,0) mode
+            // (IMEXRK.h:68-74) has a condition number ~N3^2 * (L3/dz)^2;
+            // it is solved in FP64 so that q(0,0) is not left with FP32
+            // rounding amplified by that conditioning.
+            TriFactor<double, M> F;
+            double ta[M], tb[M], tc[M];
+            T fa[M], fb[M], fc[M];
+            build_rows<T, M>(3, A, row, kk, kl, zero_mode, T(0), fa, fb, fc);
 #pragma unroll
-        for (int t = 0; t < M; ++t) {
-            u1[t] = csub(u1[t], cik(kx, dv[t]));
-            if (three) u2[t] = csub(u2[t], cik(ky, dv[t]));
-            u3[t] = csub(u3[t], cscale(coefz(ZC_DZND, t), csub(dv[t], qm[t])));
-            pp[t] = cadd(pp[t], cscale(A.pmult, dv[t]));
+            for (int t = 0; t < M; ++t) {
+                ta[t] = fa[t];
+                tb[t] = fb[t];
+                tc[t] = fc[t];
+            }
+            tri_factor<double, M>(F, ta, tb, tc, lane);
+            cx<double> dd[M];
+#pragma unroll
+            for (int t = 0; t < M; ++t) dd[t] = mkc<double>(double(dv[t].x), double(dv[t].y));
+            tri_apply<double, M>(F, dd, lane);
+            // finish the projection in FP64 for this mode as well
+            cx<double> qm[M];
+            nb_prev<double, M>(dd, qm, lane);
+#pragma unroll
+            for (int t = 0; t < M; ++t) {
+                const double dzq = double(coefz(ZC_DZND, t)) * (dd[t].x - qm[t].x);
+                const double dzqi = double(coefz(ZC_DZND, t)) * (dd[t].y - qm[t].y);
+                u3[t].x = T(double(u3[t].x) - dzq);
+                u3[t].y = T(double(u3[t].y) - dzqi);
+                pp[t].x = T(double(pp[t].x) + double(A.pmult) * dd[t].x);
+                pp[t].y = T(double(pp[t].y) + double(A.pmult) * dd[t].y);
+            }
+            // kx = ky = 0: u1 and u2 are unchanged
+        } else {
+            TriFactor<T, M> F;
+            T ta[M], tb[M], tc[M];
+            build_rows<T, M>(3, A, row, kk, kl, zero_mode, T(0), ta, tb, tc);
+            tri_factor<T, M>(F, ta, tb, tc, lane);
+            tri_apply<T, M>(F, dv, lane);  // dv now holds q
+            V qm[M];
+            nb_prev<T, M>(dv, qm, lane);
+#pragma unroll
+            for (int t = 0; t < M; ++t) {
+                u1[t] = csub(u1[t], cik(kx, dv[t]));
+                if (three) u2[t] = csub(u2[t], cik(ky, dv[t]));
+                u3[t] = csub(u3[t], cscale(coefz(ZC_DZND, t), csub(dv[t], qm[t])));
+                pp[t] = cadd(pp[t], cscale(A.pmult, dv[t]));
+            }
         }
     }
     store(A.u1, u1);
@@ -894,6 +931,51 @@ hipError_t launch_dot(const cx<T>* a, const cx<T>* b, int nmodes, int N3, const 
     return hipGetLastError();
 }
 
+// max |f + add[z]| over a nodal field (NodalField::Max, Field.h:771-786)
+template <typename T>
+__global__ __launch_bounds__(BS) void k_maxabs_partial(const T* __restrict__ f, size_t n, int N3,
+                                                       const T* __restrict__ add, double* __restrict__ partial) {
+    __shared__ double red[BS];
+    double m = 0;
+    for (size_t e = size_t(blockIdx.x) * BS + threadIdx.x; e < n; e += size_t(gridDim.x) * BS) {
+        T v = f[e];
+        if (add) v += add[e % size_t(N3)];
+        const double a = fabs(double(v));
+        m = a > m ? a : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(BS) void k_max_final(const double* __restrict__ partial, int n, double* out, int slot) {
+    __shared__ double red[BS];
+    double m = 0;
+    for (int i = threadIdx.x; i < n; i += BS) m = fmax(m, partial[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[slot] = red[0];
+}
+
+template <typename T>
+hipError_t launch_maxabs(const T* f, size_t n, int N3, const T* add, double* partial, double* out, int slot,
+                         hipStream_t s) {
+    int blocks = int((n + BS - 1) / BS);
+    if (blocks > DOT_BLOCKS) blocks = DOT_BLOCKS;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL((k_maxabs_partial<T>), dim3(blocks), dim3(BS), 0, s, f, n, N3, add, partial);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(BS), 0, s, partial, blocks, out, slot);
+    return hipGetLastError();
+}
+
 // y = a*x + b*y over n complex values (x may equal y)
 template <typename T>
 __global__ __launch_bounds__(BS) void k_axpby(cx<T>* y, T a, const cx<T>* x, T b, size_t n) {
@@ -981,6 +1063,7 @@ hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, cons
     template hipError_t launch_dot<T>(const cx<T>*, const cx<T>*, int, int, const T*, double*, double*, int,         \
                                       hipStream_t);                                                                  \
     template hipError_t launch_axpby<T>(cx<T>*, T, const cx<T>*, T, size_t, hipStream_t);                          \
+    template hipError_t launch_maxabs<T>(const T*, size_t, int, const T*, double*, double*, int, hipStream_t);       \
     template hipError_t launch_ends<T>(cx<T>*, int, int, int, hipStream_t);                                          \
     template hipError_t launch_dmul<T>(cx<T>*, const cx<T>*, int, int, int, const T*, const T*, int, const T*, int, \
                                        hipStream_t);
