@@ -117,6 +117,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=40.0, help="budget for the CPU baseline sample")
     ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_c2.json"))
+    ap.add_argument("--calibration-copies", type=int, default=0,
+                    help="device-to-device copies of one modal field (known bytes) after the timed region, "
+                         "to scale FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the IMEX-RK step is not sharded across GPUs in this build "
@@ -143,6 +146,14 @@ def main():
     s.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3
     ms_step = total_ms / args.steps
+    if args.calibration_copies:
+        a = s.field(sf.MODAL, sf.NEUMANN)
+        b = s.field(sf.MODAL, sf.NEUMANN)
+        for _ in range(args.calibration_copies):
+            b.assign(a)
+        s.synchronize()
+        _, _, Ba, _, _ = sizes(GRID["N1"], GRID["N2"], GRID["N3"])
+        print("calibration: %d copies of %d bytes" % (args.calibration_copies, Ba), file=sys.stderr)
     ke, pe = s.energies()
     if not (np.isfinite(ke) and np.isfinite(pe)):
         raise RuntimeError("non-finite energy after the timed steps")

@@ -101,6 +101,8 @@ struct Ctx {
     sf_ctx* owner = nullptr;
     int N1 = 0, N2 = 0, N3 = 0, M1 = 0, A1 = 0, A2 = 0, dim = 0, N3p = 0;
     bool three = false;
+    int zt = 4;    // z tile of the x-spectral intermediates
+    int zch = 4;   // z levels per x-pass block
     double L1 = 0, L2 = 0, L3 = 0, Re = 0, Ri = 0, Pr = 0;
     double dt = 0.01;
     T h[3] = {0, 0, 0};
@@ -188,10 +190,22 @@ struct Ctx {
             SF_REQUIRE(N2 == 1, SF_ERR_ARG, "2D requires N2 == 1 (IMEXRK.h:43)");
         }
         SF_REQUIRE(!f.evolve_background, SF_ERR_UNSUPPORTED, "EvolveBackground is not supported");
+        SF_REQUIRE(!(N1 > 384 && sizeof(T) == 8), SF_ERR_UNSUPPORTED, "FP64 contexts support N1 <= 384");
         M1 = N1 / 2 + 1;
         A1 = N1 / 3;
         A2 = three ? 2 * (N2 / 3) - 1 : 1;
         N3p = N3;
+        // z tile of the intermediates: largest of 16, 8, 4 dividing N3 whose
+        // paired forward y pass (2*zt lines) fits the y kernels' LDS budget
+        const int prec = int(sizeof(T));
+        zt = 4;
+        for (int c : {16, 8}) {
+            if (N3 % c == 0 && (!three || 2 * c <= ypass_max_lines(N2, prec))) {
+                zt = c;
+                break;
+            }
+        }
+        zch = xpass_levels(N2, N3);
         nmodal = size_t(A1) * A2 * N3;
         nnodal = size_t(N1) * N2 * N3;
         SF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -317,13 +331,14 @@ struct Ctx {
         SF_HIP(hipMemcpyAsync(kx2, hkx2.data(), A1 * sizeof(T), hipMemcpyHostToDevice, stream));
         SF_HIP(hipMemcpyAsync(ky, hky.data(), A2 * sizeof(T), hipMemcpyHostToDevice, stream));
         SF_HIP(hipMemcpyAsync(ky2, hky2.data(), A2 * sizeof(T), hipMemcpyHostToDevice, stream));
-        // twiddles exp(-2 pi i k/N), computed in double
+        // FFT twiddles, computed in double
         auto tw = [&](int n, V* dst) {
+            std::vector<double> re(n), im(n);
+            fft_twiddles(n, re.data(), im.data());
             std::vector<V> w(n);
             for (int k = 0; k < n; ++k) {
-                const double a = -2.0 * M_PI * double(k) / double(n);
-                w[k].x = T(std::cos(a));
-                w[k].y = T(std::sin(a));
+                w[k].x = T(re[k]);
+                w[k].y = T(im[k]);
             }
             SF_HIP(hipMemcpyAsync(dst, w.data(), n * sizeof(V), hipMemcpyHostToDevice, stream));
             SF_HIP(hipStreamSynchronize(stream));
@@ -359,6 +374,12 @@ struct Ctx {
 
     const V* src(int i) { return use_snap ? snap[i] : mf(SF_U1 + i); }
 
+    Lay modal_lay() const { return Lay{N3, -1, 0, 0, size_t(N3)}; }
+    Lay tiled_lay() const {
+        const int sh = zt == 16 ? 4 : (zt == 8 ? 3 : 2);
+        return Lay{zt, sh, size_t(A1) * N3, size_t(A1) * zt, size_t(zt)};
+    }
+
     // nonlinear (or linearised) explicit terms of the current state into nl[]
     void transform_path(bool linear) {
         if (three) {
@@ -367,6 +388,7 @@ struct Ctx {
             a.A1 = A1;
             a.A2 = A2;
             a.nf = 4;
+            a.lay = tiled_lay();
             for (int i = 0; i < 4; ++i) a.in[i] = src(i);
             for (int i = 0; i < 4; ++i) a.out[i] = Y[i];
             a.tw = tw2;
@@ -388,8 +410,9 @@ struct Ctx {
             x.N3 = N3;
             x.N3p = N3p;
             x.A1 = A1;
-            x.in_n2_stride = size_t(A1) * N3;
-            x.out_n2_stride = size_t(A1) * N3;
+            x.zch = zch;
+            x.lin = tiled_lay();
+            x.lout = tiled_lay();
             for (int i = 0; i < 4; ++i) x.in[i] = Y[i];
             for (int i = 0; i < 8; ++i) x.out[i] = X[i];
             x.zc = zc;
@@ -406,6 +429,7 @@ struct Ctx {
             f.A2 = A2;
             f.nf = 4;
             f.paired = 1;
+            f.lay = tiled_lay();
             for (int r = 0; r < 4; ++r) {
                 f.in[2 * r] = X[r];
                 f.in[2 * r + 1] = X[4 + r];
@@ -421,8 +445,9 @@ struct Ctx {
             x.N3 = N3;
             x.N3p = N3p;
             x.A1 = A1;
-            x.in_n2_stride = 0;
-            x.out_n2_stride = 0;
+            x.zch = zch;
+            x.lin = modal_lay();
+            x.lout = modal_lay();
             x.in[0] = src(0);
             x.in[1] = nullptr;
             x.in[2] = src(2);
@@ -524,22 +549,24 @@ struct Ctx {
         x.tw = tw1;
         x.nodal_out = nodal;
         x.scale = T(1);
+        x.nzt = 16;
         if (three) {
             YArgs<T> a{};
             a.N3 = N3;
             a.A1 = A1;
             a.A2 = A2;
             a.nf = 1;
+            a.lay = tiled_lay();
             a.in[0] = modal;
             a.out[0] = X[0];
             a.tw = tw2;
             a.scale = T(1);
             check(launch_yinv<T>(a, N2, stream), "to_nodal:y");
             x.spec_in = X[0];
-            x.spec_n2_stride = size_t(A1) * N3;
+            x.lay = tiled_lay();
         } else {
             x.spec_in = modal;
-            x.spec_n2_stride = 0;
+            x.lay = modal_lay();
         }
         check(launch_x_c2r<T>(x, N1, N2, stream), "to_nodal:x");
     }
@@ -551,9 +578,10 @@ struct Ctx {
         x.A1 = A1;
         x.tw = tw1;
         x.nodal_in = nodal;
+        x.nzt = 16;
         if (three) {
             x.spec_out = X[0];
-            x.spec_n2_stride = size_t(A1) * N3;
+            x.lay = tiled_lay();
             x.scale = T(1);
             check(launch_x_r2c<T>(x, N1, N2, stream), "to_modal:x");
             YArgs<T> a{};
@@ -562,6 +590,7 @@ struct Ctx {
             a.A2 = A2;
             a.nf = 1;
             a.paired = 0;
+            a.lay = tiled_lay();
             a.in[0] = X[0];
             a.out[0] = modal;
             a.tw = tw2;
@@ -570,7 +599,7 @@ struct Ctx {
             check(launch_yfwd<T>(a, N2, stream), "to_modal:y");
         } else {
             x.spec_out = modal;
-            x.spec_n2_stride = 0;
+            x.lay = modal_lay();
             x.scale = T(1.0 / double(N1));
             check(launch_x_r2c<T>(x, N1, N2, stream), "to_modal:x");
         }

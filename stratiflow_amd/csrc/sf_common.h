@@ -86,6 +86,14 @@ __host__ __device__ __forceinline__ V caxpy(T s, V b, V a) {
     r.y = a.y + s * b.y;
     return r;
 }
+// Workgroup barrier for LDS traffic only.  __syncthreads() also waits for
+// every outstanding global load and store of the wave (vmcnt(0)), which would
+// expose the full HBM latency at each barrier of a level-marching loop; here
+// global loads and stores stay in flight and the compiler's own waits before
+// the first use of a loaded register remain in place.  The "memory" clobber
+// keeps the compiler from moving LDS accesses across the barrier.
+__device__ __forceinline__ void sync_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 template <typename V>
 __host__ __device__ __forceinline__ V czero() {
     V r;

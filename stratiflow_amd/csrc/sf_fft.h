@@ -1,20 +1,71 @@
 // sf_fft.h -- batched complex FFTs staged in LDS for gfx950.
 //
-// A batch of NL lines of length N lives in LDS (line stride LD).  Each radix-R
-// Stockham pass loads its butterflies into registers, synchronises, and stores
-// them back in autosorted order, so a single LDS buffer suffices.  The radix
-// plan (8, then 4, 2, 3, 5) is fixed at compile time from N.  Twiddles come
-// from a table w[k] = exp(-2*pi*i*k/N) that the host fills in double precision.
-// SIGN = -1 is FFTW_FORWARD, +1 FFTW_BACKWARD; both are unnormalised.
+// A batch of NL lines of length N lives in LDS at line stride fft_ldp<N>().
+// Lines are in natural order (element i at offset i) before and after a
+// transform.  N = R1 * R2 (R1 = 16 or 8, R2 <= 32) takes two Stockham passes:
+//   pass 1: butterfly j < R2 reads x[j + R2*r], r < R1, computes a DFT-R1 in
+//           registers and writes it to row j of a padded R2 x (R1+1) table, so
+//           that the stride-R1 stores do not collide in the LDS banks;
+//   pass 2: butterfly j < R1 reads the table entries (r, j), r < R2, applies
+//           the twiddles w^(r*j), computes a DFT-R2 and writes x[j + R1*r].
+// Every LDS access is a compile-time offset from one per-thread base.  N <= 32
+// is a single DFT-N.  The register DFTs are Cooley-Tukey compositions of
+// 2/3/4/6/8-point kernels with compile-time twiddles; the pass-2 twiddles come
+// from a table tw[j*R2 + r] = exp(-2*pi*i*r*j/N) that the host fills in double
+// precision (fft_twiddle_table), so that each butterfly reads its twiddles at
+// compile-time offsets from one base.  SIGN = -1 is FFTW_FORWARD, +1 FFTW_BACKWARD; both unnormalised.
 #pragma once
 #include "sf_common.h"
 
+#include <cmath>
+
 namespace sf {
 
-constexpr int pick_radix(int rem) {
-    return (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : (rem % 5 == 0) ? 5 : rem;
+// ---- compile-time cos(2*pi*m/R) -------------------------------------------
+constexpr double cpi = 3.14159265358979323846264338327950288;
+constexpr double csin_series(double x) {  // |x| <= pi/4
+    double term = x, sum = x;
+    for (int n = 1; n < 12; ++n) {
+        term *= -x * x / double((2 * n) * (2 * n + 1));
+        sum += term;
+    }
+    return sum;
 }
+constexpr double ccos_series(double x) {  // |x| <= pi/4
+    double term = 1, sum = 1;
+    for (int n = 1; n < 12; ++n) {
+        term *= -x * x / double((2 * n - 1) * (2 * n));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double ccos2pi(int m, int R) {
+    m = ((m % R) + R) % R;
+    const double a = 2 * cpi * double(m) / double(R);
+    if (8 * m <= R) return ccos_series(a);
+    if (8 * m <= 3 * R) return csin_series(cpi / 2 - a);
+    if (8 * m <= 5 * R) return -ccos_series(cpi - a);
+    if (8 * m <= 7 * R) return csin_series(a - 3 * cpi / 2);
+    return ccos_series(2 * cpi - a);
+}
+// sin(2 pi m / R) = cos(2 pi m / R - pi/2)
+constexpr double csin2pi(int m, int R) { return ccos2pi(4 * m - R, 4 * R); }
 
+template <int R>
+struct TwTab {
+    double c[R];
+    double s[R];
+    constexpr TwTab() : c(), s() {
+        for (int m = 0; m < R; ++m) {
+            c[m] = ccos2pi(m, R);
+            s[m] = csin2pi(m, R);
+        }
+    }
+};
+template <int R>
+inline constexpr TwTab<R> kTw{};
+
+// ---- register DFTs ---------------------------------------------------------
 template <typename V, typename T, int SIGN>
 __device__ __forceinline__ V mul_si(V a) {  // (SIGN*i) * a
     V r;
@@ -23,25 +74,73 @@ __device__ __forceinline__ V mul_si(V a) {  // (SIGN*i) * a
     return r;
 }
 
-template <typename T, int R, int SIGN>
-__device__ __forceinline__ void small_dft(cx<T>* v, const cx<T>* __restrict__ tw, int N) {
+// a * exp(SIGN * 2*pi*i*m/R); m is a constant once the caller is unrolled
+template <typename T, int SIGN, int R>
+__device__ __forceinline__ cx<T> twv(cx<T> a, int m) {
+    m = m % R;
+    if (m == 0) return a;
+    if (4 * m == R) return mul_si<cx<T>, T, SIGN>(a);
+    if (2 * m == R) return mkc<T>(-a.x, -a.y);
+    if (4 * m == 3 * R) return mul_si<cx<T>, T, -SIGN>(a);
+    const T c = T(kTw<R>.c[m]);
+    const T s = T(SIGN) * T(kTw<R>.s[m]);
+    return mkc<T>(c * a.x - s * a.y, c * a.y + s * a.x);
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft3(cx<T>& a0, cx<T>& a1, cx<T>& a2) {
     using V = cx<T>;
-    if constexpr (R == 2) {
+    const T s3 = T(SIGN) * T(0.86602540378443864676);
+    const V t1 = cadd(a1, a2);
+    const V t2 = csub(a0, cscale(T(0.5), t1));
+    const V t3 = cscale(s3, csub(a1, a2));
+    V it3;
+    it3.x = -t3.y;
+    it3.y = t3.x;
+    a0 = cadd(a0, t1);
+    a1 = cadd(t2, it3);
+    a2 = csub(t2, it3);
+}
+
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void reg_dft(cx<T>* v);
+
+// R = P*Q in registers: n = Q*n1 + n2, k = k1 + P*k2
+template <typename T, int P, int Q, int SIGN>
+__device__ __forceinline__ void reg_dft_ct(cx<T>* v) {
+    using V = cx<T>;
+    constexpr int R = P * Q;
+    V t[R];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) {
+        V c[P];
+#pragma unroll
+        for (int n1 = 0; n1 < P; ++n1) c[n1] = v[Q * n1 + n2];
+        reg_dft<T, P, SIGN>(c);
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) t[k1 * Q + n2] = twv<T, SIGN, R>(c[k1], n2 * k1);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+        V c[Q];
+#pragma unroll
+        for (int n2 = 0; n2 < Q; ++n2) c[n2] = t[k1 * Q + n2];
+        reg_dft<T, Q, SIGN>(c);
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) v[k1 + P * k2] = c[k2];
+    }
+}
+
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void reg_dft(cx<T>* v) {
+    using V = cx<T>;
+    if constexpr (R == 1) {
+    } else if constexpr (R == 2) {
         V a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     } else if constexpr (R == 3) {
-        const T s3 = T(SIGN) * T(0.86602540378443864676);
-        V t1 = cadd(v[1], v[2]);
-        V t2 = csub(v[0], cscale(T(0.5), t1));
-        V d = csub(v[1], v[2]);
-        V t3 = cscale(s3, d);
-        v[0] = cadd(v[0], t1);
-        V it3;
-        it3.x = -t3.y;
-        it3.y = t3.x;
-        v[1] = cadd(t2, it3);
-        v[2] = csub(t2, it3);
+        dft3<T, SIGN>(v[0], v[1], v[2]);
     } else if constexpr (R == 4) {
         V a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
         V a2 = cadd(v[1], v[3]), a3 = mul_si<V, T, SIGN>(csub(v[1], v[3]));
@@ -49,21 +148,30 @@ __device__ __forceinline__ void small_dft(cx<T>* v, const cx<T>* __restrict__ tw
         v[2] = csub(a0, a2);
         v[1] = cadd(a1, a3);
         v[3] = csub(a1, a3);
+    } else if constexpr (R == 6) {
+        // 6 = 2 x 3: E = DFT3(v0,v2,v4), O = DFT3(v1,v3,v5), y_k = E_k + W6^k O_k
+        V e0 = v[0], e1 = v[2], e2 = v[4];
+        V o0 = v[1], o1 = v[3], o2 = v[5];
+        dft3<T, SIGN>(e0, e1, e2);
+        dft3<T, SIGN>(o0, o1, o2);
+        const V w1o1 = twv<T, SIGN, 6>(o1, 1);
+        const V w2o2 = twv<T, SIGN, 6>(o2, 2);
+        v[0] = cadd(e0, o0);
+        v[3] = csub(e0, o0);
+        v[1] = cadd(e1, w1o1);
+        v[4] = csub(e1, w1o1);
+        v[2] = cadd(e2, w2o2);
+        v[5] = csub(e2, w2o2);
     } else if constexpr (R == 8) {
-        const T c = T(0.70710678118654752440);
         V e0 = cadd(v[0], v[4]), e1 = csub(v[0], v[4]);
         V e2 = cadd(v[2], v[6]), e3 = mul_si<V, T, SIGN>(csub(v[2], v[6]));
         V E0 = cadd(e0, e2), E2 = csub(e0, e2), E1 = cadd(e1, e3), E3 = csub(e1, e3);
         V o0 = cadd(v[1], v[5]), o1 = csub(v[1], v[5]);
         V o2 = cadd(v[3], v[7]), o3 = mul_si<V, T, SIGN>(csub(v[3], v[7]));
         V O0 = cadd(o0, o2), O2 = csub(o0, o2), O1 = cadd(o1, o3), O3 = csub(o1, o3);
-        // W8^1 = c(1 + SIGN i), W8^2 = SIGN i, W8^3 = c(-1 + SIGN i)
-        V w1O1, w2O2, w3O3;
-        w1O1.x = c * (O1.x - T(SIGN) * O1.y);
-        w1O1.y = c * (O1.y + T(SIGN) * O1.x);
-        w2O2 = mul_si<V, T, SIGN>(O2);
-        w3O3.x = c * (-O3.x - T(SIGN) * O3.y);
-        w3O3.y = c * (-O3.y + T(SIGN) * O3.x);
+        const V w1O1 = twv<T, SIGN, 8>(O1, 1);
+        const V w2O2 = mul_si<V, T, SIGN>(O2);
+        const V w3O3 = twv<T, SIGN, 8>(O3, 3);
         v[0] = cadd(E0, O0);
         v[4] = csub(E0, O0);
         v[1] = cadd(E1, w1O1);
@@ -72,87 +180,182 @@ __device__ __forceinline__ void small_dft(cx<T>* v, const cx<T>* __restrict__ tw
         v[6] = csub(E2, w2O2);
         v[3] = cadd(E3, w3O3);
         v[7] = csub(E3, w3O3);
+    } else if constexpr (R == 12) {
+        reg_dft_ct<T, 4, 3, SIGN>(v);
+    } else if constexpr (R == 16) {
+        reg_dft_ct<T, 4, 4, SIGN>(v);
+    } else if constexpr (R == 24) {
+        reg_dft_ct<T, 8, 3, SIGN>(v);
+    } else if constexpr (R == 32) {
+        reg_dft_ct<T, 8, 4, SIGN>(v);
     } else {
-        // generic small prime: O(R^2) DFT with twiddles from the length-N table
-        V t[R];
-        const int step = N / R;
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-            V acc = czero<V>();
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                V w = tw[((r * s) % R) * step];
-                if (SIGN > 0) w.y = -w.y;
-                acc = cadd(acc, cmul(v[r], w));
-            }
-            t[s] = acc;
-        }
-#pragma unroll
-        for (int s = 0; s < R; ++s) v[s] = t[s];
+        static_assert(R == 0, "unsupported register DFT size");
     }
 }
 
-// One Stockham pass over NL lines.  Called by all BS threads of the block.
-template <typename T, int N, int R, int NS, int NL, int LD, int BS, int SIGN>
-__device__ __forceinline__ void fft_pass(cx<T>* buf, const cx<T>* __restrict__ tw) {
+// ---- plan ------------------------------------------------------------------
+template <int N>
+struct FftPlan {
+    static constexpr bool two = N > 32;
+    static constexpr int R1 = !two ? N : ((N % 16 == 0 && N / 16 <= 32) ? 16 : 8);
+    static constexpr int R2 = N / R1;
+    static_assert(!two || (R1 * R2 == N && R2 <= 32), "unsupported FFT length");
+};
+
+// LDS line stride: room for the padded pass-1 table, staggered by half a bank
+// cycle (16 complex values) between consecutive lines
+template <int N>
+constexpr int fft_ldp() {
+    if constexpr (N <= 32) {
+        return N + 1;
+    } else {
+        int l = N + FftPlan<N>::R2;
+        while (l % 32 != 16) ++l;
+        return l;
+    }
+}
+
+template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
+__device__ __forceinline__ void fft_pass1(int nl, LP lp) {
     using V = cx<T>;
-    constexpr int NB = N / R;
-    constexpr int TOT = NL * NB;
-    constexpr int PER = (TOT + BS - 1) / BS;
-    constexpr int TSTEP = N / (NS * R);
-    V v[PER][R];
+    constexpr int R1 = FftPlan<N>::R1, R2 = FftPlan<N>::R2;
+    constexpr int PER = (NLMAX * R2 + BS - 1) / BS;
+    const int tot = nl * R2;
+    V v[PER][R1];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
         const int g = int(threadIdx.x) + i * BS;
-        if ((TOT % BS == 0) || g < TOT) {
-            const int line = g / NB;
-            const int j = g - line * NB;
-            const int k = j % NS;
-            const V* L = buf + line * LD;
+        if (g < tot) {
+            const int line = g / R2;
+            const int j = g - line * R2;
+            const V* L = lp(line) + j;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[i][r] = L[j + r * NB];
-            if constexpr (NS > 1) {
+            for (int r = 0; r < R1; ++r) v[i][r] = L[R2 * r];
+            reg_dft<T, R1, SIGN>(v[i]);
+        }
+    }
+    sync_lds();
 #pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    V w = tw[r * k * TSTEP];
-                    if (SIGN > 0) w.y = -w.y;
-                    v[i][r] = cmul(v[i][r], w);
-                }
+    for (int i = 0; i < PER; ++i) {
+        const int g = int(threadIdx.x) + i * BS;
+        if (g < tot) {
+            const int line = g / R2;
+            const int j = g - line * R2;
+            V* Lp = lp(line) + j * (R1 + 1);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) Lp[r] = v[i][r];
+        }
+    }
+    sync_lds();
+}
+
+// pass 2 twiddles: tw[j*R2 + r] = exp(-2 pi i r j / N), j < R1, r < R2
+template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
+__device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict__ tw) {
+    using V = cx<T>;
+    constexpr int R1 = FftPlan<N>::R1, R2 = FftPlan<N>::R2;
+    constexpr int PER = (NLMAX * R1 + BS - 1) / BS;
+    const int tot = nl * R1;
+    V v[PER][R2];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int g = int(threadIdx.x) + i * BS;
+        if (g < tot) {
+            const int line = g / R1;
+            const int j = g - line * R1;
+            const V* Lp = lp(line) + j;
+            const V* twj = tw + j * R2;
+#pragma unroll
+            for (int r = 0; r < R2; ++r) v[i][r] = Lp[r * (R1 + 1)];
+#pragma unroll
+            for (int r = 1; r < R2; ++r) {
+                V w = twj[r];
+                if (SIGN > 0) w.y = -w.y;
+                v[i][r] = cmul(v[i][r], w);
             }
-            small_dft<T, R, SIGN>(v[i], tw, N);
+            reg_dft<T, R2, SIGN>(v[i]);
         }
     }
-    __syncthreads();
+    sync_lds();
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
         const int g = int(threadIdx.x) + i * BS;
-        if ((TOT % BS == 0) || g < TOT) {
-            const int line = g / NB;
-            const int j = g - line * NB;
-            const int k = j % NS;
-            V* L = buf + line * LD;
-            const int o = (j / NS) * NS * R + k;
+        if (g < tot) {
+            const int line = g / R1;
+            const int j = g - line * R1;
+            V* L = lp(line) + j;
 #pragma unroll
-            for (int r = 0; r < R; ++r) L[o + r * NS] = v[i][r];
+            for (int r = 0; r < R2; ++r) L[R1 * r] = v[i][r];
         }
     }
-    __syncthreads();
+    sync_lds();
 }
 
-template <typename T, int N, int NS, int NL, int LD, int BS, int SIGN>
-__device__ __forceinline__ void fft_run(cx<T>* buf, const cx<T>* __restrict__ tw) {
-    if constexpr (NS < N) {
-        constexpr int R = pick_radix(N / NS);
-        fft_pass<T, N, R, NS, NL, LD, BS, SIGN>(buf, tw);
-        fft_run<T, N, NS * R, NL, LD, BS, SIGN>(buf, tw);
+template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
+__device__ __forceinline__ void fft_single(int nl, LP lp) {
+    using V = cx<T>;
+    constexpr int PER = (NLMAX + BS - 1) / BS;
+    V v[PER][N];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int line = int(threadIdx.x) + i * BS;
+        if (line < nl) {
+            const V* L = lp(line);
+#pragma unroll
+            for (int r = 0; r < N; ++r) v[i][r] = L[r];
+            reg_dft<T, N, SIGN>(v[i]);
+        }
+    }
+    sync_lds();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int line = int(threadIdx.x) + i * BS;
+        if (line < nl) {
+            V* L = lp(line);
+#pragma unroll
+            for (int r = 0; r < N; ++r) L[r] = v[i][r];
+        }
+    }
+    sync_lds();
+}
+
+// In-place FFT of nl <= NLMAX lines in LDS, each in natural order at lp(line)
+// with room for fft_ldp<N>() values.  The caller must have synchronised after
+// filling the lines; on return the results are visible to all threads.
+template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
+__device__ __forceinline__ void lds_fft_lines(int nl, LP lp, const cx<T>* __restrict__ tw) {
+    if constexpr (FftPlan<N>::two) {
+        fft_pass1<T, N, NLMAX, BS, SIGN>(nl, lp);
+        fft_pass2<T, N, NLMAX, BS, SIGN>(nl, lp, tw);
+    } else {
+        fft_single<T, N, NLMAX, BS, SIGN>(nl, lp);
+        (void)tw;
     }
 }
 
-// In-place FFT of NL lines in LDS.  The caller must have synchronised after
-// filling buf; on return the results are visible to all threads.
-template <typename T, int N, int NL, int LD, int BS, int SIGN>
+// NL contiguous lines at stride fft_ldp<N>()
+template <typename T, int N, int NL, int BS, int SIGN>
 __device__ __forceinline__ void lds_fft(cx<T>* buf, const cx<T>* __restrict__ tw) {
-    fft_run<T, N, 1, NL, LD, BS, SIGN>(buf, tw);
+    constexpr int LDP = fft_ldp<N>();
+    lds_fft_lines<T, N, NL, BS, SIGN>(NL, [buf](int line) { return buf + line * LDP; }, tw);
+}
+
+// host: twiddle table of length N in the layout fft_pass2 reads (entry count N)
+template <int N>
+void fft_twiddle_table(double* re, double* im) {
+    constexpr int R1 = FftPlan<N>::R1, R2 = FftPlan<N>::R2;
+    for (int k = 0; k < N; ++k) {
+        re[k] = 1.0;
+        im[k] = 0.0;
+    }
+    if constexpr (FftPlan<N>::two) {
+        for (int j = 0; j < R1; ++j)
+            for (int r = 0; r < R2; ++r) {
+                const double a = -2.0 * cpi * double((r * j) % N) / double(N);
+                re[j * R2 + r] = std::cos(a);
+                im[j * R2 + r] = std::sin(a);
+            }
+    }
 }
 
 }  // namespace sf

@@ -18,25 +18,41 @@ struct ModalArgs {
     int mode;  // 0: full stage, 1: projection only (RemoveDivergence)
 };
 
+// Position of element (n2, j1, z) of an x-spectral field:
+//   n2*s_n2 + (z / zt)*s_zb + j1*s_j1 + z % zt
+// 3D intermediates are z-tiled ([n2][z/zt][j1][z%zt], zt = 16 when N3 allows)
+// so that an x-pass block reads and writes whole contiguous tiles; modal
+// fields are the special case zt = N3, s_j1 = N3 ([j1][z]).
+struct Lay {
+    int zt;   // tile height (a power of two), or N3 for the untiled modal layout
+    int zsh;  // log2(zt), or -1 when untiled
+    size_t s_n2, s_zb, s_j1;
+};
+__host__ __device__ inline size_t lay_at(const Lay& L, int n2, int j1, int z) {
+    if (L.zsh < 0) return size_t(n2) * L.s_n2 + size_t(j1) * L.s_j1 + size_t(z);
+    return size_t(n2) * L.s_n2 + size_t(z >> L.zsh) * L.s_zb + size_t(j1) * L.s_j1 + size_t(z & (L.zt - 1));
+}
+
 // Fused inverse-x / products / forward-x kernel.
 template <typename T>
 struct XArgs {
-    int N3, N3p, A1, nzt;
-    size_t in_n2_stride, out_n2_stride;  // elements between consecutive n2 planes
-    const cx<T>* in[4];                  // u1, u2, u3, b  ([n2][j1][z])
+    int N3, N3p, A1, zch;                // zch: z levels per block (a multiple of the sub-tile)
+    Lay lin, lout;
+    const cx<T>* in[4];                  // u1, u2, u3, b spectra
     cx<T>* out[8];                       // 3D: A1,A2,A3,AB,B1,B2,B3,BB ; 2D: Nl1,-,Nl3,NlB
     const T* zc;
     const T* kx;
     const cx<T>* tw;
     T scale;
     int linear;                          // 1: linearised products (BuildRHSLinear)
-    const cx<T>* base[4];                // linear: u1t, u2t, u3t, bt spectra ([n2][j1][z])
+    const cx<T>* base[4];                // linear: u1t, u2t, u3t, bt spectra (layout lin)
 };
 
-// y-direction transforms (3D): per (j1, z-tile).
+// y-direction transforms (3D): per (j1, z-tile of lay.zt levels).
 template <typename T>
 struct YArgs {
-    int N3, A1, A2, nzt, nf;
+    int N3, A1, A2, nf;
+    Lay lay;          // layout of the x-spectral side
     const cx<T>* in[8];
     cx<T>* out[8];
     const T* ky;      // A2 active wavenumbers (forward: i*ky*B term)
@@ -49,7 +65,7 @@ struct YArgs {
 template <typename T>
 struct XTArgs {
     int N3, A1, nzt;
-    size_t spec_n2_stride;
+    Lay lay;
     const cx<T>* spec_in;
     cx<T>* spec_out;
     const T* nodal_in;
@@ -85,6 +101,10 @@ hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, cons
                        const T* zc, int N3p, hipStream_t s);
 
 bool tz_supported(int n);
+int xpass_levels(int N2, int N3);
+// twiddle table of an n-point LDS FFT (the layout its second pass reads)
+void fft_twiddles(int n, double* re, double* im);
+int ypass_max_lines(int N2, int precision);
 int max_rows_per_lane();
 
 }  // namespace sf

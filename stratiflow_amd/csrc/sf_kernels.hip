@@ -354,243 +354,16 @@ int max_rows_per_lane() { return 8; }
 
 // ---------------------------------------------------------------------------
 // k_xpass: inverse x FFT, products, forward x FFT
+//
+// A block owns one spanwise plane n2 and A.zch consecutive z levels, which it
+// visits one level at a time.  A ring of three nodal levels (z-1, z, z+1, the
+// +-1 halo of the vertical stencils) stays in LDS.  Each step computes all
+// products of level z from the ring, then runs ONE batched FFT over the
+// product lines of level z and the spectra of level z+2 (which replaces z-1 in
+// the ring): every level is inverse-transformed once per block and no
+// separate halo pass is needed.  The inverse transform uses
+// ifft(x) = conj(fft(conj(x))), so both directions share the forward kernel.
 // ---------------------------------------------------------------------------
-template <int N, typename T, int NI>
-constexpr int xpass_tz() {
-    // LDS: NI*(TZ+2) + 3*TZ lines of N complex values, kept under ~80 KB
-    constexpr int bytes = 80 * 1024;
-    constexpr int lines = bytes / (N * int(sizeof(cx<T>)));
-    constexpr int tz = (lines - 2 * NI) / (NI + 3);
-    return tz < 1 ? 1 : (tz > 8 ? 8 : tz);
-}
-
-template <typename T, int N1, int TZ, bool THREED, bool LIN>
-__global__ __launch_bounds__(BS) void k_xpass(XArgs<T> A) {
-    using V = cx<T>;
-    constexpr int NLEV = TZ + 2;
-    constexpr int NI = LIN ? 4 : 2;      // inverse pair lines per level
-    constexpr int NLI = NI * NLEV;
-    constexpr int NLP = 3 * TZ;          // product pair lines per half
-    __shared__ V bufI[NLI * N1];
-    __shared__ V bufP[NLP * N1];
-
-    const int zt = blockIdx.x;
-    const int n2 = blockIdx.y;
-    const int z0 = zt * TZ;
-    const int N3 = A.N3, A1 = A.A1;
-    const int tid = threadIdx.x;
-
-    // 1) Hermitian-extended spectra, two real fields per complex line
-    for (int idx = tid; idx < NLI * N1; idx += BS) {
-        const int lev = idx % NLEV;
-        const int rest = idx / NLEV;
-        const int j = rest % N1;
-        const int pair = rest / N1;
-        const int zz = z0 - 1 + lev;
-        V val = czero<V>();
-        if (zz >= 0 && zz < N3) {
-            int jj = -1;
-            bool cj = false;
-            if (j < A1) jj = j;
-            else if (j > N1 - A1) {
-                jj = N1 - j;
-                cj = true;
-            }
-            if (jj >= 0) {
-                const int src = pair & 1;
-                const bool bas = pair >= 2;
-                const V* const* in = bas ? A.base : A.in;
-                const V* fa;
-                const V* fb;
-                if (src == 0) {
-                    fa = in[0];
-                    fb = in[2];
-                } else if (THREED) {
-                    fa = in[1];
-                    fb = in[3];
-                } else {
-                    fa = in[3];
-                    fb = nullptr;
-                }
-                const size_t off = size_t(n2) * A.in_n2_stride + size_t(jj) * N3 + zz;
-                V X = fa[off];
-                V Y = fb ? fb[off] : czero<V>();
-                if (jj == 0) {  // c2r ignores the imaginary part of the DC term
-                    X.y = 0;
-                    Y.y = 0;
-                }
-                if (cj) {
-                    X.y = -X.y;
-                    Y.y = -Y.y;
-                }
-                val.x = X.x - Y.y;
-                val.y = X.y + Y.x;
-            }
-        }
-        bufI[(pair * NLEV + lev) * N1 + j] = val;
-    }
-    __syncthreads();
-    lds_fft<T, N1, NLI, N1, BS, +1>(bufI, A.tw);
-
-    // nodal accessors (zero outside 0..N3-1 because those lines were zero)
-    auto nod = [&](int pair, int lev, int n, bool im) -> T {
-        const V v = bufI[(pair * NLEV + lev) * N1 + n];
-        return im ? v.y : v.x;
-    };
-    auto ubar = [&](int j) -> T { return (j >= 0 && j < N3) ? A.zc[ZC_UBAR * A.N3p + j] : T(0); };
-
-    const int nhalf = THREED ? 2 : 1;
-    for (int half = 0; half < nhalf; ++half) {
-        // 2) products for TZ levels
-        for (int idx = tid; idx < TZ * N1; idx += BS) {
-            const int n = idx % N1;
-            const int t = idx / N1;
-            const int j = z0 + t;
-            const int lc = t + 1, lm = t, lp = t + 2;
-            V o0 = czero<V>(), o1 = czero<V>(), o2 = czero<V>();
-            if (j < N3) {
-                const T itl = A.zc[ZC_ITIL_LO * A.N3p + j], itd = A.zc[ZC_ITIL_DI * A.N3p + j];
-                const T dDN = A.zc[ZC_DZDN * A.N3p + j], dND = A.zc[ZC_DZND * A.N3p + j];
-                const T hc = (j >= 1) ? T(0.5) : T(0);  // ReinterpolateBar row j
-                const T u3m = nod(0, lm, n, true), u3c = nod(0, lc, n, true), u3p = nod(0, lp, n, true);
-                const T B_m = nod(1, lm, n, THREED), B_c = nod(1, lc, n, THREED), B_p = nod(1, lp, n, THREED);
-                // total streamwise velocity U1 + U_
-                const T u1m = nod(0, lm, n, false) + ubar(j - 1);
-                const T u1c = nod(0, lc, n, false) + ubar(j);
-                const T u1p = nod(0, lp, n, false) + ubar(j + 1);
-                T u2m = 0, u2c = 0, u2p = 0;
-                if (THREED) {
-                    u2m = nod(1, lm, n, false);
-                    u2c = nod(1, lc, n, false);
-                    u2p = nod(1, lp, n, false);
-                }
-                if (!LIN) {
-                    if (half == 0) {
-                        // P1 = U1t^2 ; P2 = ddz(Ibar(U1t)*U3)          (IMEXRK.cpp:155-159)
-                        const T P1 = u1c * u1c;
-                        const T gc = (hc * u1m + hc * u1c) * u3c, gp = (T(0.5) * u1c + T(0.5) * u1p) * u3p;
-                        const T P2 = (gp - gc) * dDN;
-                        // P3 = Itilde(U1t)*U3 ; Q4 = ddz(Idir(U3)^2)    (IMEXRK.cpp:161-163)
-                        const T P3 = (itl * u1m + itd * u1c) * u3c;
-                        const T ic = T(0.5) * u3c + T(0.5) * u3p, im = T(0.5) * u3m + T(0.5) * u3c;
-                        const T Q4 = (ic * ic - im * im) * dND;
-                        // P5 = ddz(Ibar(B)*U3) ; P6 = U1t*B            (IMEXRK.cpp:182-192)
-                        const T gbc = (hc * B_m + hc * B_c) * u3c, gbp = (T(0.5) * B_c + T(0.5) * B_p) * u3p;
-                        const T P5 = (gbp - gbc) * dDN;
-                        const T P6 = u1c * B_c;
-                        o0 = mkc<T>(P1, P2);
-                        o1 = mkc<T>(P3, Q4);
-                        o2 = mkc<T>(P5, P6);
-                    } else {
-                        // P7 = U2^2 ; P8 = ddz(Ibar(U2)*U3)             (IMEXRK.cpp:167-171)
-                        const T P7 = u2c * u2c;
-                        const T gc = (hc * u2m + hc * u2c) * u3c, gp = (T(0.5) * u2c + T(0.5) * u2p) * u3p;
-                        const T P8 = (gp - gc) * dDN;
-                        // P9 = Itilde(U2)*U3 ; P10 = U1t*U2 ; P11 = U2*B (IMEXRK.cpp:173-188)
-                        const T P9 = (itl * u2m + itd * u2c) * u3c;
-                        const T P10 = u1c * u2c;
-                        const T P11 = u2c * B_c;
-                        o0 = mkc<T>(P7, P8);
-                        o1 = mkc<T>(P9, P10);
-                        o2 = mkc<T>(P11, T(0));
-                    }
-                } else {
-                    // BuildRHSLinear, IMEXRK.cpp:195-258; base state U*_tot, nnTemp = U1_tot + U_
-                    const T v3m = nod(2, lm, n, true), v3c = nod(2, lc, n, true), v3p = nod(2, lp, n, true);
-                    const T w1m = nod(2, lm, n, false) + ubar(j - 1);
-                    const T w1c = nod(2, lc, n, false) + ubar(j);
-                    const T w1p = nod(2, lp, n, false) + ubar(j + 1);
-                    const T Bb_m = nod(3, lm, n, THREED), Bb_c = nod(3, lc, n, THREED), Bb_p = nod(3, lp, n, THREED);
-                    T w2m = 0, w2c = 0, w2p = 0;
-                    if (THREED) {
-                        w2m = nod(3, lm, n, false);
-                        w2c = nod(3, lc, n, false);
-                        w2p = nod(3, lp, n, false);
-                    }
-                    // perturbation U1 without background
-                    const T p1m = nod(0, lm, n, false), p1c = nod(0, lc, n, false), p1p = nod(0, lp, n, false);
-                    (void)u1m;
-                    (void)u1c;
-                    (void)u1p;
-                    if (half == 0) {
-                        const T P1 = T(2) * (p1c * w1c);
-                        const T gc = (hc * p1m + hc * p1c) * v3c + (hc * w1m + hc * w1c) * u3c;
-                        const T gp = (T(0.5) * p1c + T(0.5) * p1p) * v3p + (T(0.5) * w1c + T(0.5) * w1p) * u3p;
-                        const T P2 = (gp - gc) * dDN;
-                        const T P3 = (itl * p1m + itd * p1c) * v3c + (itl * w1m + itd * w1c) * u3c;
-                        const T ic = (T(0.5) * u3c + T(0.5) * u3p) * (T(0.5) * v3c + T(0.5) * v3p);
-                        const T im = (T(0.5) * u3m + T(0.5) * u3c) * (T(0.5) * v3m + T(0.5) * v3c);
-                        const T Q4 = T(2) * ((ic - im) * dND);
-                        const T gbc = (hc * B_m + hc * B_c) * v3c + (hc * Bb_m + hc * Bb_c) * u3c;
-                        const T gbp = (T(0.5) * B_c + T(0.5) * B_p) * v3p + (T(0.5) * Bb_c + T(0.5) * Bb_p) * u3p;
-                        const T P5 = (gbp - gbc) * dDN;
-                        const T P6 = B_c * w1c + Bb_c * p1c;
-                        o0 = mkc<T>(P1, P2);
-                        o1 = mkc<T>(P3, Q4);
-                        o2 = mkc<T>(P5, P6);
-                    } else {
-                        const T P7 = T(2) * (u2c * w2c);
-                        const T gc = (hc * u2m + hc * u2c) * v3c + (hc * w2m + hc * w2c) * u3c;
-                        const T gp = (T(0.5) * u2c + T(0.5) * u2p) * v3p + (T(0.5) * w2c + T(0.5) * w2p) * u3p;
-                        const T P8 = (gp - gc) * dDN;
-                        const T P9 = (itl * u2m + itd * u2c) * v3c + (itl * w2m + itd * w2c) * u3c;
-                        const T P10 = p1c * w2c + w1c * u2c;
-                        const T P11 = B_c * w2c + Bb_c * u2c;
-                        o0 = mkc<T>(P7, P8);
-                        o1 = mkc<T>(P9, P10);
-                        o2 = mkc<T>(P11, T(0));
-                    }
-                }
-            }
-            bufP[(0 * TZ + t) * N1 + n] = o0;
-            bufP[(1 * TZ + t) * N1 + n] = o1;
-            bufP[(2 * TZ + t) * N1 + n] = o2;
-        }
-        __syncthreads();
-        lds_fft<T, N1, NLP, N1, BS, -1>(bufP, A.tw);
-
-        // 3) unpack the two real spectra of each line, keep k < A1, combine
-        for (int idx = tid; idx < TZ * A1; idx += BS) {
-            const int t = idx % TZ;
-            const int k = idx / TZ;
-            const int j = z0 + t;
-            if (j >= N3) continue;
-            const T kxv = A.kx[k];
-            V Fa[3], Fb[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const V Z = bufP[(q * TZ + t) * N1 + k];
-                const V W = bufP[(q * TZ + t) * N1 + ((N1 - k) % N1)];
-                // Fa = (Z + conj W)/2 ; Fb = (Z - conj W)/(2i)
-                Fa[q].x = T(0.5) * (Z.x + W.x);
-                Fa[q].y = T(0.5) * (Z.y - W.y);
-                Fb[q].x = T(0.5) * (Z.y + W.y);
-                Fb[q].y = T(-0.5) * (Z.x - W.x);
-            }
-            const size_t off = size_t(n2) * A.out_n2_stride + size_t(k) * N3 + j;
-            const T sc = A.scale;
-            if (half == 0) {
-                // r1 -= ikx P1 + P2 ; r3 -= ikx P3 + ddz P4 ; rB -= P5 + ikx P6
-                const V r1 = cadd(cik(kxv, Fa[0]), Fb[0]);
-                const V r3 = cadd(cik(kxv, Fa[1]), Fb[1]);
-                const V rb = cadd(Fa[2], cik(kxv, Fb[2]));
-                A.out[0][off] = cscale(-sc, r1);
-                A.out[2][off] = cscale(-sc, r3);
-                A.out[3][off] = cscale(-sc, rb);
-            } else {
-                // r2 -= iky P7 + P8 + ikx P10 ; r3 -= iky P9 ; r1 -= iky P10 ; rB -= iky P11
-                const V a2 = cadd(Fb[0], cik(kxv, Fb[1]));
-                A.out[1][off] = cscale(-sc, a2);
-                A.out[4][off] = cscale(-sc, Fb[1]);
-                A.out[5][off] = cscale(-sc, Fa[0]);
-                A.out[6][off] = cscale(-sc, Fa[1]);
-                A.out[7][off] = cscale(-sc, Fa[2]);
-            }
-        }
-        __syncthreads();
-    }
-}
-
 #define SF_SIZES(X) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
 
 bool tz_supported(int n) {
@@ -605,24 +378,248 @@ bool tz_supported(int n) {
 }
 
 template <typename T, int N1, bool THREED, bool LIN>
-static hipError_t launch_xpass_n(const XArgs<T>& a, int N2, hipStream_t s) {
-    constexpr int TZ = xpass_tz<N1, T, LIN ? 4 : 2>();
-    XArgs<T> b = a;
-    b.nzt = (a.N3 + TZ - 1) / TZ;
-    dim3 grid(b.nzt, THREED ? N2 : 1);
-    hipLaunchKernelGGL((k_xpass<T, N1, TZ, THREED, LIN>), grid, dim3(BS), 0, s, b);
-    return hipGetLastError();
+__global__ __launch_bounds__(BS) void k_xpass(XArgs<T> A) {
+    using V = cx<T>;
+    constexpr int LDP = fft_ldp<N1>();
+    constexpr int NI = LIN ? 4 : 2;        // input pair lines per level
+    constexpr int NP = THREED ? 6 : 3;     // product pair lines per level
+    constexpr int NB = NP + NI;            // lines per batched FFT
+    constexpr int AMAX = N1 / 3;           // A1 <= N1/3
+    constexpr int FI = (NI * AMAX + BS - 1) / BS;
+    // [NP product lines][ring slot 0: NI lines][slot 1][slot 2]
+    __shared__ V lds[(NP + 3 * NI) * LDP];
+    V* const prd = lds;
+    V* const ring = lds + NP * LDP;
+
+    const int n2 = blockIdx.y;
+    const int N3 = A.N3, A1 = A.A1;
+    const int zbeg = blockIdx.x * A.zch;
+    const int zend = min(zbeg + A.zch, N3);
+    const int tid = threadIdx.x;
+    auto slot = [&](int z) { return (z + 1 - zbeg) % 3; };
+
+    // spectra of one level into registers: item = pair * A1 + j1
+    V X[FI], Y[FI];
+    auto load = [&](int z) {
+#pragma unroll
+        for (int i = 0; i < FI; ++i) {
+            const int item = tid + i * BS;
+            X[i] = czero<V>();
+            Y[i] = czero<V>();
+            if (item < NI * A1 && z >= 0 && z < N3) {
+                const int pair = item / A1;
+                const int j1 = item - pair * A1;
+                const V* const* in = (pair >= 2) ? A.base : A.in;
+                const size_t off = lay_at(A.lin, n2, j1, z);
+                if ((pair & 1) == 0) {
+                    X[i] = in[0][off];
+                    Y[i] = in[2][off];
+                } else if (THREED) {
+                    X[i] = in[1][off];
+                    Y[i] = in[3][off];
+                } else {
+                    X[i] = in[3][off];
+                }
+            }
+        }
+    };
+    // conj of the Hermitian-extended pair line Z = F(a) + i F(b) into ring slot s
+    auto store = [&](int s) {
+        V* R = ring + s * NI * LDP;
+        const int nband = N1 - 2 * A1 + 1;  // j = A1 .. N1-A1 is zero
+        for (int idx = tid; idx < NI * nband; idx += BS) {
+            const int pair = idx / nband;
+            R[pair * LDP + A1 + (idx - pair * nband)] = czero<V>();
+        }
+#pragma unroll
+        for (int i = 0; i < FI; ++i) {
+            const int item = tid + i * BS;
+            if (item < NI * A1) {
+                const int pair = item / A1;
+                const int j1 = item - pair * A1;
+                V* L = R + pair * LDP;
+                const V x = X[i], y = Y[i];
+                if (j1 == 0) {  // c2r ignores the imaginary part of the DC term
+                    L[0] = mkc<T>(x.x, -y.x);
+                } else {
+                    L[j1] = mkc<T>(x.x - y.y, -(x.y + y.x));
+                    L[N1 - j1] = mkc<T>(x.x + y.y, x.y - y.x);
+                }
+            }
+        }
+    };
+
+    // prologue: levels zbeg-1, zbeg, zbeg+1 into slots 0, 1, 2
+    for (int l = 0; l < 3; ++l) {
+        load(zbeg - 1 + l);
+        store(l);
+    }
+    sync_lds();
+    lds_fft_lines<T, N1, 3 * NI, BS, -1>(3 * NI, [ring](int line) { return ring + line * LDP; }, A.tw);
+
+    for (int z = zbeg; z < zend; ++z) {
+        const bool more = z + 1 < zend;
+        if (more) load(z + 2);  // in flight while the products are formed
+
+        // ---- all products of level z --------------------------------------
+        {
+            const T itl = A.zc[ZC_ITIL_LO * A.N3p + z], itd = A.zc[ZC_ITIL_DI * A.N3p + z];
+            const T dDN = A.zc[ZC_DZDN * A.N3p + z], dND = A.zc[ZC_DZND * A.N3p + z];
+            const T hc = (z >= 1) ? T(0.5) : T(0);  // ReinterpolateBar row z (row 0 is empty)
+            const T h = T(0.5);
+            T ub[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int zz = z - 1 + d;
+                ub[d] = (zz >= 0 && zz < N3) ? A.zc[ZC_UBAR * A.N3p + zz] : T(0);
+            }
+            const V* Ls[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) Ls[d] = ring + slot(z - 1 + d) * NI * LDP;
+            for (int n = tid; n < N1; n += BS) {
+                T u1[3], u2[3], u3[3], bb[3], w1[3], w2[3], w3[3], wb[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const V a = Ls[d][n];           // conj(U1 + i U3)
+                    const V c = Ls[d][LDP + n];     // conj(U2 + i B) or conj(B)
+                    u1[d] = a.x + (LIN ? T(0) : ub[d]);
+                    u3[d] = -a.y;
+                    u2[d] = THREED ? c.x : T(0);
+                    bb[d] = THREED ? -c.y : c.x;
+                    if (LIN) {
+                        const V e = Ls[d][2 * LDP + n];
+                        const V f = Ls[d][3 * LDP + n];
+                        w1[d] = e.x + ub[d];
+                        w3[d] = -e.y;
+                        w2[d] = THREED ? f.x : T(0);
+                        wb[d] = THREED ? -f.y : f.x;
+                    } else {
+                        w1[d] = w2[d] = w3[d] = wb[d] = T(0);
+                    }
+                }
+                T P[12];
+                if (!LIN) {
+                    // BuildRHS products (IMEXRK.cpp:136-193), U1t = U1 + U_
+                    P[0] = u1[1] * u1[1];                                                   // P1
+                    P[1] = ((h * u1[1] + h * u1[2]) * u3[2] - (hc * u1[0] + hc * u1[1]) * u3[1]) * dDN;  // P2
+                    P[2] = (itl * u1[0] + itd * u1[1]) * u3[1];                            // P3
+                    {
+                        const T ic = h * u3[1] + h * u3[2], im = h * u3[0] + h * u3[1];
+                        P[3] = (ic * ic - im * im) * dND;                                   // Q4
+                    }
+                    P[4] = ((h * bb[1] + h * bb[2]) * u3[2] - (hc * bb[0] + hc * bb[1]) * u3[1]) * dDN;  // P5
+                    P[5] = u1[1] * bb[1];                                                   // P6
+                    if (THREED) {
+                        P[6] = ((h * u2[1] + h * u2[2]) * u3[2] - (hc * u2[0] + hc * u2[1]) * u3[1]) * dDN;  // P8
+                        P[7] = u1[1] * u2[1];                                               // P10
+                        P[8] = u2[1] * u2[1];                                               // P7
+                        P[9] = (itl * u2[0] + itd * u2[1]) * u3[1];                        // P9
+                        P[10] = u2[1] * bb[1];                                              // P11
+                        P[11] = T(0);
+                    }
+                } else {
+                    // BuildRHSLinear products (IMEXRK.cpp:195-258) about the base w
+                    // (w1 includes U_); u1 is the perturbation without U_
+                    P[0] = T(2) * (u1[1] * w1[1]);
+                    {
+                        const T gc = (hc * u1[0] + hc * u1[1]) * w3[1] + (hc * w1[0] + hc * w1[1]) * u3[1];
+                        const T gp = (h * u1[1] + h * u1[2]) * w3[2] + (h * w1[1] + h * w1[2]) * u3[2];
+                        P[1] = (gp - gc) * dDN;
+                    }
+                    P[2] = (itl * u1[0] + itd * u1[1]) * w3[1] + (itl * w1[0] + itd * w1[1]) * u3[1];
+                    {
+                        const T ic = (h * u3[1] + h * u3[2]) * (h * w3[1] + h * w3[2]);
+                        const T im = (h * u3[0] + h * u3[1]) * (h * w3[0] + h * w3[1]);
+                        P[3] = T(2) * ((ic - im) * dND);
+                    }
+                    {
+                        const T gc = (hc * bb[0] + hc * bb[1]) * w3[1] + (hc * wb[0] + hc * wb[1]) * u3[1];
+                        const T gp = (h * bb[1] + h * bb[2]) * w3[2] + (h * wb[1] + h * wb[2]) * u3[2];
+                        P[4] = (gp - gc) * dDN;
+                    }
+                    P[5] = bb[1] * w1[1] + wb[1] * u1[1];
+                    if (THREED) {
+                        const T gc = (hc * u2[0] + hc * u2[1]) * w3[1] + (hc * w2[0] + hc * w2[1]) * u3[1];
+                        const T gp = (h * u2[1] + h * u2[2]) * w3[2] + (h * w2[1] + h * w2[2]) * u3[2];
+                        P[6] = (gp - gc) * dDN;
+                        P[7] = u1[1] * w2[1] + w1[1] * u2[1];
+                        P[8] = T(2) * (u2[1] * w2[1]);
+                        P[9] = (itl * u2[0] + itd * u2[1]) * w3[1] + (itl * w2[0] + itd * w2[1]) * u3[1];
+                        P[10] = bb[1] * w2[1] + wb[1] * u2[1];
+                        P[11] = T(0);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NP; ++q) prd[q * LDP + n] = mkc<T>(P[2 * q], P[2 * q + 1]);
+            }
+        }
+        sync_lds();
+        if (more) store(slot(z + 2));  // replaces level z-1
+        sync_lds();
+        {
+            const int s2 = slot(z + 2) * NI;
+            lds_fft_lines<T, N1, NB, BS, -1>(more ? NB : NP,
+                                             [prd, s2](int line) {
+                                                 return prd + (line + (line >= NP ? s2 : 0)) * LDP;
+                                             },
+                                             A.tw);
+        }
+        // ---- separate the two real spectra of each product line, combine ---
+        for (int k = tid; k < A1; k += BS) {
+            V Fa[NP], Fb[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const V Z = prd[q * LDP + k];
+                const V W = prd[q * LDP + ((N1 - k) % N1)];
+                // Fa = (Z + conj W)/2, Fb = (Z - conj W)/(2i)
+                Fa[q].x = T(0.5) * (Z.x + W.x);
+                Fa[q].y = T(0.5) * (Z.y - W.y);
+                Fb[q].x = T(0.5) * (Z.y + W.y);
+                Fb[q].y = T(-0.5) * (Z.x - W.x);
+            }
+            const T kxv = A.kx[k];
+            const T ms = -A.scale;
+            const size_t off = lay_at(A.lout, n2, k, z);
+            // r1 -= ikx P1 + P2 ; r3 -= ikx P3 + ddz P4 ; rB -= P5 + ikx P6
+            A.out[0][off] = cscale(ms, cadd(cik(kxv, Fa[0]), Fb[0]));
+            A.out[2][off] = cscale(ms, cadd(cik(kxv, Fa[1]), Fb[1]));
+            A.out[3][off] = cscale(ms, cadd(Fa[2], cik(kxv, Fb[2])));
+            if constexpr (THREED) {
+                // r2 -= P8 + ikx P10 ; r1 -= iky P10 ; r2 -= iky P7 ; r3 -= iky P9 ; rB -= iky P11
+                A.out[1][off] = cscale(ms, cadd(Fa[3], cik(kxv, Fb[3])));
+                A.out[4][off] = cscale(ms, Fb[3]);
+                A.out[5][off] = cscale(ms, Fa[4]);
+                A.out[6][off] = cscale(ms, Fb[4]);
+                A.out[7][off] = cscale(ms, Fa[5]);
+            }
+        }
+        sync_lds();
+    }
+}
+
+template <typename T, int N1, bool THREED, bool LIN>
+static hipError_t launch_xpass_v(const XArgs<T>& a, int N2, hipStream_t s) {
+    constexpr int lds = ((THREED ? 6 : 3) + 3 * (LIN ? 4 : 2)) * fft_ldp<N1>() * int(sizeof(cx<T>));
+    if constexpr (lds > 160 * 1024) {
+        return hipErrorInvalidValue;  // not instantiated: does not fit the LDS of one CU
+    } else {
+        if (a.zch < 1 || a.A1 > N1 / 3) return hipErrorInvalidValue;
+        const int nblk = (a.N3 + a.zch - 1) / a.zch;
+        dim3 grid(nblk, THREED ? N2 : 1);
+        hipLaunchKernelGGL((k_xpass<T, N1, THREED, LIN>), grid, dim3(BS), 0, s, a);
+        return hipGetLastError();
+    }
 }
 
 template <typename T>
 hipError_t launch_xpass(const XArgs<T>& a, int N1, int N2, bool threeD, hipStream_t s) {
     switch (N1) {
-#define SF_CASE(n)                                                                                     \
-    case n:                                                                                            \
-        if (a.linear)                                                                                  \
-            return threeD ? launch_xpass_n<T, n, true, true>(a, N2, s)                                 \
-                          : launch_xpass_n<T, n, false, true>(a, N2, s);                               \
-        return threeD ? launch_xpass_n<T, n, true, false>(a, N2, s) : launch_xpass_n<T, n, false, false>(a, N2, s);
+#define SF_CASE(n)                                                                                   \
+    case n:                                                                                          \
+        if (a.linear)                                                                                \
+            return threeD ? launch_xpass_v<T, n, true, true>(a, N2, s)                               \
+                          : launch_xpass_v<T, n, false, true>(a, N2, s);                             \
+        return threeD ? launch_xpass_v<T, n, true, false>(a, N2, s) : launch_xpass_v<T, n, false, false>(a, N2, s);
         SF_SIZES(SF_CASE)
 #undef SF_CASE
         default:
@@ -630,122 +627,159 @@ hipError_t launch_xpass(const XArgs<T>& a, int N1, int N2, bool threeD, hipStrea
     }
 }
 
-// ---------------------------------------------------------------------------
-// y transforms (3D)
-// ---------------------------------------------------------------------------
-template <int N, typename T>
-constexpr int ypass_tz() {
-    constexpr int lines = (48 * 1024) / (N * int(sizeof(cx<T>)));
-    constexpr int tz = lines / 2;
-    return tz < 1 ? 1 : (tz > 16 ? 16 : tz);
+// levels per x-pass block: enough blocks to fill the chip several times over,
+// at most 32 levels so that the ring prologue (two extra levels) stays cheap
+int xpass_levels(int N2, int N3) {
+    int zc = (N2 * N3) / 1024;
+    if (zc > 32) zc = 32;
+    if (zc < 2) zc = 2;
+    return zc;
 }
 
-// inverse: out_f[n2][j1][z] = sum_{j2 active} in_f[j2a][j1][z] e^{+2 pi i j2 n2/N2}
-template <typename T, int N2, int TZ>
-__global__ __launch_bounds__(BS) void k_yinv(YArgs<T> A) {
-    using V = cx<T>;
-    constexpr int NL = 2 * TZ;
-    __shared__ V buf[NL * N2];
-    const int z0 = blockIdx.x * TZ;
-    const int j1 = blockIdx.y;
-    const int N3 = A.N3, A1 = A.A1;
-    const int h = N2 / 3;
-    for (int f0 = 0; f0 < A.nf; f0 += 2) {
-        for (int idx = threadIdx.x; idx < NL * N2; idx += BS) {
-            const int t = idx % TZ;
-            const int rest = idx / TZ;
-            const int j2 = rest % N2;
-            const int fl = rest / N2;
-            const int f = f0 + fl;
-            const int z = z0 + t;
-            V v = czero<V>();
-            int j2a = -1;
-            if (j2 < h) j2a = j2;
-            else if (j2 >= N2 - h + 1) j2a = j2 - N2 + 2 * h - 1;
-            if (f < A.nf && z < N3 && j2a >= 0) v = A.in[f][(size_t(j2a) * A1 + j1) * N3 + z];
-            buf[(fl * TZ + t) * N2 + j2] = v;
-        }
-        __syncthreads();
-        lds_fft<T, N2, NL, N2, BS, +1>(buf, A.tw);
-        for (int idx = threadIdx.x; idx < NL * N2; idx += BS) {
-            const int t = idx % TZ;
-            const int rest = idx / TZ;
-            const int n2 = rest % N2;
-            const int fl = rest / N2;
-            const int f = f0 + fl;
-            const int z = z0 + t;
-            if (f < A.nf && z < N3) A.out[f][(size_t(n2) * A1 + j1) * N3 + z] = buf[(fl * TZ + t) * N2 + n2];
-        }
-        __syncthreads();
+void fft_twiddles(int n, double* re, double* im) {
+    switch (n) {
+#define SF_CASE(n)                           \
+    case n:                                  \
+        fft_twiddle_table<n>(re, im);        \
+        return;
+        SF_SIZES(SF_CASE)
+#undef SF_CASE
+        default:
+            return;
     }
 }
 
-// forward: X_f[n2][j1][z] -> active j2.  paired: out[r] = scale*(F(in[2r]) + i ky F(in[2r+1]))
-template <typename T, int N2, int TZ>
-__global__ __launch_bounds__(BS) void k_yfwd(YArgs<T> A) {
+// ---------------------------------------------------------------------------
+// y transforms (3D), one block per (z-tile of lay.zt levels, j1)
+// ---------------------------------------------------------------------------
+template <int N, typename T>
+constexpr int ypass_lines() {
+    constexpr int lines = (53 * 1024) / (fft_ldp<N>() * int(sizeof(cx<T>)));
+    return lines < 8 ? 8 : (lines > 32 ? 32 : lines);
+}
+
+// inverse: out_f(n2, j1, z) = sum_{j2 active} in_f[j2a][j1][z] e^{+2 pi i j2 n2/N2}
+template <typename T, int N2, int NL>
+__global__ __launch_bounds__(BS) void k_yinv(YArgs<T> A) {
     using V = cx<T>;
-    constexpr int NL = 2 * TZ;
-    __shared__ V buf[NL * N2];
-    const int z0 = blockIdx.x * TZ;
+    constexpr int LDP = fft_ldp<N2>();
+    __shared__ V buf[NL * LDP];
+    const int zt = A.lay.zt;
+    const int z0 = blockIdx.x * zt;
     const int j1 = blockIdx.y;
     const int N3 = A.N3, A1 = A.A1, A2 = A.A2;
     const int h = N2 / 3;
-    const int nin = A.paired ? 2 * A.nf : A.nf;
-    for (int f0 = 0; f0 < nin; f0 += 2) {
+    const int fpr = NL / zt;  // fields per round
+    for (int f0 = 0; f0 < A.nf; f0 += fpr) {
+        // zero band j2 = h .. N2-h of every line, then scatter the active modes
+        const int nband = N2 - 2 * h + 1;
+        for (int idx = threadIdx.x; idx < NL * nband; idx += BS) {
+            const int line = idx / nband;
+            buf[line * LDP + (h + idx % nband)] = czero<V>();
+        }
+        for (int idx = threadIdx.x; idx < NL * A2; idx += BS) {
+            const int t = idx % zt;
+            const int rest = idx / zt;
+            const int ja = rest % A2;
+            const int fl = rest / A2;
+            const int f = f0 + fl;
+            const int z = z0 + t;
+            V v = czero<V>();
+            if (f < A.nf && z < N3) v = A.in[f][(size_t(ja) * A1 + j1) * N3 + z];
+            const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
+            buf[(fl * zt + t) * LDP + (j2)] = v;
+        }
+        sync_lds();
+        lds_fft<T, N2, NL, BS, +1>(buf, A.tw);
         for (int idx = threadIdx.x; idx < NL * N2; idx += BS) {
-            const int t = idx % TZ;
-            const int rest = idx / TZ;
+            const int t = idx % zt;
+            const int rest = idx / zt;
+            const int n2 = rest % N2;
+            const int fl = rest / N2;
+            const int f = f0 + fl;
+            const int z = z0 + t;
+            if (f < A.nf && z < N3) A.out[f][lay_at(A.lay, n2, j1, z)] = buf[(fl * zt + t) * LDP + (n2)];
+        }
+        sync_lds();
+    }
+}
+
+// forward: X_f(n2, j1, z) -> active j2.  paired: out[r] = scale*(F(in[2r]) + i ky F(in[2r+1]))
+template <typename T, int N2, int NL>
+__global__ __launch_bounds__(BS) void k_yfwd(YArgs<T> A) {
+    using V = cx<T>;
+    constexpr int LDP = fft_ldp<N2>();
+    __shared__ V buf[NL * LDP];
+    const int zt = A.lay.zt;
+    const int z0 = blockIdx.x * zt;
+    const int j1 = blockIdx.y;
+    const int N3 = A.N3, A1 = A.A1, A2 = A.A2;
+    const int h = N2 / 3;
+    const int fpr = NL / zt;  // input fields per round (even when paired)
+    const int nin = A.paired ? 2 * A.nf : A.nf;
+    for (int f0 = 0; f0 < nin; f0 += fpr) {
+        for (int idx = threadIdx.x; idx < NL * N2; idx += BS) {
+            const int t = idx % zt;
+            const int rest = idx / zt;
             const int n2 = rest % N2;
             const int fl = rest / N2;
             const int f = f0 + fl;
             const int z = z0 + t;
             V v = czero<V>();
-            if (f < nin && z < N3) v = A.in[f][(size_t(n2) * A1 + j1) * N3 + z];
-            buf[(fl * TZ + t) * N2 + n2] = v;
+            if (f < nin && z < N3) v = A.in[f][lay_at(A.lay, n2, j1, z)];
+            buf[(fl * zt + t) * LDP + (n2)] = v;
         }
-        __syncthreads();
-        lds_fft<T, N2, NL, N2, BS, -1>(buf, A.tw);
+        sync_lds();
+        lds_fft<T, N2, NL, BS, -1>(buf, A.tw);
         if (A.paired) {
-            const int r = f0 / 2;
-            for (int idx = threadIdx.x; idx < TZ * A2; idx += BS) {
-                const int t = idx % TZ;
-                const int j2a = idx / TZ;
+            for (int idx = threadIdx.x; idx < (fpr / 2) * zt * A2; idx += BS) {
+                const int t = idx % zt;
+                const int rest = idx / zt;
+                const int ja = rest % A2;
+                const int pr = rest / A2;
+                const int r = f0 / 2 + pr;
                 const int z = z0 + t;
-                if (z >= N3) continue;
-                const int j2 = (j2a < h) ? j2a : j2a + N2 - 2 * h + 1;
-                const V Fa = buf[(0 * TZ + t) * N2 + j2];
-                const V Fb = buf[(1 * TZ + t) * N2 + j2];
-                const V v = cadd(Fa, cik(A.ky[j2a], Fb));
-                A.out[r][(size_t(j2a) * A1 + j1) * N3 + z] = cscale(A.scale, v);
+                if (r >= A.nf || z >= N3) continue;
+                const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
+                const V Fa = buf[((2 * pr) * zt + t) * LDP + (j2)];
+                const V Fb = buf[((2 * pr + 1) * zt + t) * LDP + (j2)];
+                A.out[r][(size_t(ja) * A1 + j1) * N3 + z] = cscale(A.scale, cadd(Fa, cik(A.ky[ja], Fb)));
             }
         } else {
-            for (int idx = threadIdx.x; idx < 2 * TZ * A2; idx += BS) {
-                const int t = idx % TZ;
-                const int rest = idx / TZ;
-                const int j2a = rest % A2;
+            for (int idx = threadIdx.x; idx < fpr * zt * A2; idx += BS) {
+                const int t = idx % zt;
+                const int rest = idx / zt;
+                const int ja = rest % A2;
                 const int fl = rest / A2;
                 const int f = f0 + fl;
                 const int z = z0 + t;
                 if (f >= nin || z >= N3) continue;
-                const int j2 = (j2a < h) ? j2a : j2a + N2 - 2 * h + 1;
-                A.out[f][(size_t(j2a) * A1 + j1) * N3 + z] = cscale(A.scale, buf[(fl * TZ + t) * N2 + j2]);
+                const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
+                A.out[f][(size_t(ja) * A1 + j1) * N3 + z] = cscale(A.scale, buf[(fl * zt + t) * LDP + (j2)]);
             }
         }
-        __syncthreads();
+        sync_lds();
     }
 }
 
 template <typename T, int N2>
 static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
-    constexpr int TZ = ypass_tz<N2, T>();
-    YArgs<T> b = a;
-    b.nzt = (a.N3 + TZ - 1) / TZ;
-    dim3 grid(b.nzt, a.A1);
-    if (inv)
-        hipLaunchKernelGGL((k_yinv<T, N2, TZ>), grid, dim3(BS), 0, s, b);
-    else
-        hipLaunchKernelGGL((k_yfwd<T, N2, TZ>), grid, dim3(BS), 0, s, b);
-    return hipGetLastError();
+    constexpr int maxl = ypass_lines<N2, T>();
+    const int zt = a.lay.zt;
+    // lines per block: a whole number of fields (pairs for the paired forward)
+    const int unit = (!inv && a.paired) ? 2 * zt : zt;
+    if (unit > maxl) return hipErrorInvalidValue;
+    dim3 grid((a.N3 + zt - 1) / zt, a.A1);
+    const int want = (maxl / unit) * unit;
+#define SF_NL(L)                                                                              \
+    if (want >= L && L % unit == 0) {                                                         \
+        if (inv) hipLaunchKernelGGL((k_yinv<T, N2, (L < maxl ? L : maxl)>), grid, dim3(BS), 0, s, a); \
+        else hipLaunchKernelGGL((k_yfwd<T, N2, (L < maxl ? L : maxl)>), grid, dim3(BS), 0, s, a);     \
+        return hipGetLastError();                                                             \
+    }
+    SF_NL(32) SF_NL(16) SF_NL(8) SF_NL(4) SF_NL(2)
+#undef SF_NL
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -773,82 +807,97 @@ hipError_t launch_yfwd(const YArgs<T>& a, int N2, hipStream_t s) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// single-field x transforms (ToNodal / ToModal for API fields)
-// ---------------------------------------------------------------------------
-template <typename T, int N1, int TZ>
-__global__ __launch_bounds__(BS) void k_x_c2r(XTArgs<T> A) {
-    using V = cx<T>;
-    __shared__ V buf[TZ * N1];
-    const int z0 = blockIdx.x * TZ;
-    const int n2 = blockIdx.y;
-    const int N3 = A.N3, A1 = A.A1;
-    for (int idx = threadIdx.x; idx < TZ * N1; idx += BS) {
-        const int t = idx % TZ;
-        const int j = idx / TZ;
-        const int z = z0 + t;
-        V v = czero<V>();
-        if (z < N3) {
-            int jj = -1;
-            bool cj = false;
-            if (j < A1) jj = j;
-            else if (j > N1 - A1) {
-                jj = N1 - j;
-                cj = true;
-            }
-            if (jj >= 0) {
-                v = A.spec_in[size_t(n2) * A.spec_n2_stride + size_t(jj) * N3 + z];
-                if (jj == 0) v.y = 0;
-                if (cj) v.y = -v.y;
-            }
-        }
-        buf[t * N1 + j] = v;
-    }
-    __syncthreads();
-    lds_fft<T, N1, TZ, N1, BS, +1>(buf, A.tw);
-    for (int idx = threadIdx.x; idx < TZ * N1; idx += BS) {
-        const int t = idx % TZ;
-        const int n = idx / TZ;
-        const int z = z0 + t;
-        if (z < N3) A.nodal_out[(size_t(n2) * N1 + n) * N3 + z] = buf[t * N1 + n].x;
+int ypass_max_lines(int N2, int precision) {
+    switch (N2) {
+#define SF_CASE(n) \
+    case n:        \
+        return precision == 4 ? ypass_lines<n, float>() : ypass_lines<n, double>();
+        SF_SIZES(SF_CASE)
+#undef SF_CASE
+        default:
+            return 0;
     }
 }
 
-template <typename T, int N1, int TZ>
-__global__ __launch_bounds__(BS) void k_x_r2c(XTArgs<T> A) {
+// ---------------------------------------------------------------------------
+// single-field x transforms (ToNodal / ToModal for API fields), one block per
+// (z-tile of lay.zt levels, n2)
+// ---------------------------------------------------------------------------
+template <typename T, int N1, int NL>
+__global__ __launch_bounds__(BS) void k_x_c2r(XTArgs<T> A) {
     using V = cx<T>;
-    __shared__ V buf[TZ * N1];
-    const int z0 = blockIdx.x * TZ;
+    constexpr int LDP = fft_ldp<N1>();
+    __shared__ V buf[NL * LDP];
     const int n2 = blockIdx.y;
     const int N3 = A.N3, A1 = A.A1;
-    for (int idx = threadIdx.x; idx < TZ * N1; idx += BS) {
-        const int t = idx % TZ;
-        const int n = idx / TZ;
-        const int z = z0 + t;
-        V v = czero<V>();
-        if (z < N3) v.x = A.nodal_in[(size_t(n2) * N1 + n) * N3 + z];
-        buf[t * N1 + n] = v;
+    for (int z0 = blockIdx.x * A.nzt; z0 < min(N3, (int(blockIdx.x) + 1) * A.nzt); z0 += NL) {
+        const int nband = N1 - 2 * A1 + 1;
+        for (int idx = threadIdx.x; idx < NL * nband; idx += BS)
+            buf[(idx / nband) * LDP + (A1 + idx % nband)] = czero<V>();
+        for (int idx = threadIdx.x; idx < NL * A1; idx += BS) {
+            const int t = idx % NL;
+            const int j1 = idx / NL;
+            const int z = z0 + t;
+            V X = czero<V>();
+            if (z < N3) X = A.spec_in[lay_at(A.lay, n2, j1, z)];
+            V* L = buf + t * LDP;
+            if (j1 == 0) {
+                L[(0)] = mkc<T>(X.x, T(0));
+            } else {
+                L[(j1)] = X;
+                L[(N1 - j1)] = mkc<T>(X.x, -X.y);
+            }
+        }
+        sync_lds();
+        lds_fft<T, N1, NL, BS, +1>(buf, A.tw);
+        for (int idx = threadIdx.x; idx < NL * N1; idx += BS) {
+            const int t = idx % NL;
+            const int n = idx / NL;
+            const int z = z0 + t;
+            if (z < N3) A.nodal_out[(size_t(n2) * N1 + n) * N3 + z] = buf[t * LDP + (n)].x;
+        }
+        sync_lds();
     }
-    __syncthreads();
-    lds_fft<T, N1, TZ, N1, BS, -1>(buf, A.tw);
-    for (int idx = threadIdx.x; idx < TZ * A1; idx += BS) {
-        const int t = idx % TZ;
-        const int k = idx / TZ;
-        const int z = z0 + t;
-        if (z < N3) A.spec_out[size_t(n2) * A.spec_n2_stride + size_t(k) * N3 + z] = cscale(A.scale, buf[t * N1 + k]);
+}
+
+template <typename T, int N1, int NL>
+__global__ __launch_bounds__(BS) void k_x_r2c(XTArgs<T> A) {
+    using V = cx<T>;
+    constexpr int LDP = fft_ldp<N1>();
+    __shared__ V buf[NL * LDP];
+    const int n2 = blockIdx.y;
+    const int N3 = A.N3, A1 = A.A1;
+    for (int z0 = blockIdx.x * A.nzt; z0 < min(N3, (int(blockIdx.x) + 1) * A.nzt); z0 += NL) {
+        for (int idx = threadIdx.x; idx < NL * N1; idx += BS) {
+            const int t = idx % NL;
+            const int n = idx / NL;
+            const int z = z0 + t;
+            V v = czero<V>();
+            if (z < N3) v.x = A.nodal_in[(size_t(n2) * N1 + n) * N3 + z];
+            buf[t * LDP + (n)] = v;
+        }
+        sync_lds();
+        lds_fft<T, N1, NL, BS, -1>(buf, A.tw);
+        for (int idx = threadIdx.x; idx < NL * A1; idx += BS) {
+            const int t = idx % NL;
+            const int k = idx / NL;
+            const int z = z0 + t;
+            if (z < N3) A.spec_out[lay_at(A.lay, n2, k, z)] = cscale(A.scale, buf[t * LDP + (k)]);
+        }
+        sync_lds();
     }
 }
 
 template <typename T, int N1>
 static hipError_t launch_xt_n(const XTArgs<T>& a, int N2, bool c2r, hipStream_t s) {
-    constexpr int TZ = ypass_tz<N1, T>();
+    constexpr int NL = 4;  // z levels per LDS batch; a block sweeps a.nzt levels
     XTArgs<T> b = a;
-    b.nzt = (a.N3 + TZ - 1) / TZ;
-    dim3 grid(b.nzt, N2);
+    if (b.nzt % NL) b.nzt = ((b.nzt + NL - 1) / NL) * NL;
+    dim3 grid((a.N3 + b.nzt - 1) / b.nzt, N2);
     if (c2r)
-        hipLaunchKernelGGL((k_x_c2r<T, N1, TZ>), grid, dim3(BS), 0, s, b);
+        hipLaunchKernelGGL((k_x_c2r<T, N1, NL>), grid, dim3(BS), 0, s, b);
     else
-        hipLaunchKernelGGL((k_x_r2c<T, N1, TZ>), grid, dim3(BS), 0, s, b);
+        hipLaunchKernelGGL((k_x_r2c<T, N1, NL>), grid, dim3(BS), 0, s, b);
     return hipGetLastError();
 }
 

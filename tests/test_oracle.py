@@ -345,3 +345,47 @@ def test_fp32_oracle_tracks_fp64_oracle():
         out.append(o.energy())
     assert abs(out[1][0] - out[0][0]) / out[0][0] < 1e-4
     assert abs(out[1][1] - out[0][1]) / out[0][1] < 1e-4
+
+
+# ---------------------------------------------------------------- BuildRHSLinear
+@pytest.mark.parametrize("dim,N2", [(O.TWO_D, 1), (O.THREE_D, 12)])
+def test_linearised_rhs_is_the_derivative(dim, N2):
+    """BuildRHSLinear about w applied to v equals the derivative of BuildRHS at w in
+    direction v.  The products are quadratic, so the central difference
+    (R(w+ev) - R(w-ev)) / 2e is exact up to rounding."""
+    from stratiflow_amd import ic
+    N1, N3 = 24, 32
+    o0 = make(N1=N1, N2=N2, N3=N3, dim=dim)
+    z, zf, _, _ = o0.grid()
+    U = ic.tanh_background(zf)
+    w = ic.perturbation(N1, N2, N3, dim, z, zf, 5, amplitude=0.2)
+    v = ic.perturbation(N1, N2, N3, dim, z, zf, 6, amplitude=0.1)
+    rn = ["r1", "r3", "rB"] + (["r2"] if dim == O.THREE_D else [])
+    eps = 1e-3
+
+    def r_nonlinear(sign):
+        o = make(N1=N1, N2=N2, N3=N3, dim=dim)
+        o.set_background(U)
+        for n in w:
+            o.set_modal(n, w[n] + sign * eps * v[n])
+        o.filter_all()
+        o.populate_nodal()
+        o.stage(0)
+        return {n: o.get_modal(n) for n in rn}
+
+    rp, rm = r_nonlinear(+1), r_nonlinear(-1)
+    o = make(N1=N1, N2=N2, N3=N3, dim=dim)
+    o.set_background(U)
+    for n in v:
+        o.set_modal(n, v[n])
+    names = {"u1": "u1t", "u2": "u2t", "u3": "u3t", "b": "bt"}
+    for n in w:
+        o.set_modal(names[n], w[n])
+    o.set_linear_background()
+    o.filter_all()
+    o.populate_nodal()
+    o.stage(0, linear=True)
+    for n in rn:
+        fd = (rp[n] - rm[n]) / (2 * eps)
+        lin = o.get_modal(n)
+        assert np.linalg.norm(fd - lin) / np.linalg.norm(lin) < 1e-9, n
