@@ -23,6 +23,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <random>
+#include <sys/stat.h>
 #include <vector>
 
 #include "../sfhip.h"
@@ -59,6 +61,29 @@ struct FlowParams {  // Parameters.h:21-30
 
 inline stratifloat InitialU(stratifloat z) { return std::tanh(z); }  // Parameters.h:33-36
 
+// physicists' Hermite polynomial H_n(x) (Field.cpp:91-131)
+inline stratifloat Hermite(unsigned int n, stratifloat x) {
+    double hm = 1, hc = 2 * double(x);
+    if (n == 0) return 1;
+    for (unsigned int m = 1; m < n; ++m) {
+        const double hn = 2 * double(x) * hc - 2 * double(m) * hm;
+        hm = hc;
+        hc = hn;
+    }
+    return stratifloat(hc);
+}
+
+// mkdir -p (OSUtils.cpp:14-70)
+inline void MakeCleanDir(const std::string& dir) {
+    std::string path;
+    for (size_t i = 0; i <= dir.size(); ++i) {
+        if (i == dir.size() || (dir[i] == '/' && i > 0)) {
+            path = dir.substr(0, i);
+            mkdir(path.c_str(), S_IRWXU);
+        }
+    }
+}
+
 namespace sfacade {
 
 template <typename D = void>
@@ -67,6 +92,7 @@ struct Globals {
     static FlowParams flow;
     static sf_ctx* ctx;
     static bool ctx_three;
+    static std::mt19937* rng;
 };
 // defaults of Parameters.h:42-43 and Parameters.cpp:6-7
 template <typename D>
@@ -78,6 +104,16 @@ template <typename D>
 sf_ctx* Globals<D>::ctx = nullptr;
 template <typename D>
 bool Globals<D>::ctx_three = false;
+template <typename D>
+std::mt19937* Globals<D>::rng = nullptr;
+
+// the generator behind ExciteLowWavenumbers; the reference draws a fresh
+// std::random_device seed on every call (Field.h:1074-1076), the facade keeps
+// one generator that SeedRandom makes reproducible
+inline std::mt19937& rng() {
+    if (!Globals<>::rng) Globals<>::rng = new std::mt19937(std::random_device{}());
+    return *Globals<>::rng;
+}
 
 inline void check(int rc, const sf_ctx* c) {
     if (rc != SF_OK) throw std::runtime_error(std::string("sfhip: ") + sf_last_error(c));
@@ -241,6 +277,8 @@ public:
     }
 
     void ToNodal(NodalField& other) const;
+    // low-wavenumber Hermite-Gaussian excitation (Field.h:1072-1116)
+    void ExciteLowWavenumbers(stratifloat L);
     void ZeroEnds() { sfacade::check(sf_field_zero_ends(dev(true)), sfacade::ctx()); }
     void NeumannEnds() { sfacade::check(sf_field_neumann_ends(dev(true)), sfacade::ctx()); }
     void Filter() { sfacade::check(sf_field_filter(dev(true)), sfacade::ctx()); }
@@ -392,6 +430,41 @@ inline void VerticalPointsBoth(std::vector<double>& z, std::vector<double>& zf) 
     zf.assign(gridParams.N3, 0.0);
     sfacade::check(sf_vertical_points(sfacade::ctx(), z.data(), zf.data()), sfacade::ctx());
 }
+
+// Field.h:1072-1116: coefficients (j1^2+j2^2+h^2+1)^(-5/3) (U(-1,1) + i U(-1,1))
+// on the profiles exp(-2 z^2) H_h(2 z), h < 4, for j1 < N1/6 and |j2| <= M2
+// (M2 = N2/6 in 3D, N2/2 otherwise), drawn from the facade's generator.
+// The context's L3 (flowParams.L3) fixes the vertical points.
+inline void ModalField::ExciteLowWavenumbers(stratifloat L) {
+    (void)L;
+    std::vector<double> z, zf;
+    VerticalPointsBoth(z, zf);
+    const std::vector<double>& zz = (bc_ == BoundaryCondition::Dirichlet) ? z : zf;
+    const GridParams& g = gridParams;
+    const int M2 = g.ThirdDimension() ? g.N2 / 6 : g.N2 / 2;
+    std::uniform_real_distribution<stratifloat> dist(-1.0, 1.0);
+    std::mt19937& gen = sfacade::rng();
+    for (int j1 = 0; j1 < g.N1 / 6; j1++) {
+        for (int j2 = -M2; j2 <= M2; j2++) {
+            complex coeff[4];
+            for (int h = 0; h < 4; h++) {
+                const stratifloat re = dist(gen);
+                const stratifloat im = dist(gen);
+                coeff[h] = stratifloat(std::pow(double(j1 * j1 + j2 * j2 + h * h + 1), -5.0 / 3.0)) * complex(re, im);
+            }
+            const int actualj2 = j2 < 0 ? j2 + g.N2 : j2;
+            for (int j3 = 0; j3 < g.N3; j3++) {
+                complex v = 0;
+                const stratifloat zj = stratifloat(zz[j3]);
+                for (int h = 0; h < 4; h++) v += coeff[h] * std::exp(-zj * zj * 2) * Hermite(h, zj * 2);
+                (*this)(j1, actualj2, j3) = v;
+            }
+        }
+    }
+}
+
+// reproducible ExciteLowWavenumbers draws
+inline void SeedRandom(unsigned int seed) { sfacade::rng().seed(seed); }
 
 inline void Nodal1D::SetValue(std::function<stratifloat(stratifloat)> f, stratifloat L3) {
     (void)L3;  // the context's L3 is flowParams.L3
@@ -755,6 +828,15 @@ public:
         solver().PopulateNodalVariables();
         solver().RemoveDivergence(0.0f);
         CopyFromSolver();
+    }
+
+    // StateVector.cpp:227-235
+    void ExciteLowWavenumbers(stratifloat energy) {
+        u1.ExciteLowWavenumbers(flowParams.L3);
+        u2.ExciteLowWavenumbers(flowParams.L3);
+        u3.ExciteLowWavenumbers(flowParams.L3);
+        b.ExciteLowWavenumbers(flowParams.L3);
+        Rescale(energy);
     }
 
     void SaveToFile(const std::string& filename) const {  // StateVector.h:260-273
