@@ -176,6 +176,27 @@ int sf_time_steps(sf_ctx* ctx, int nsteps, double* ms);
 #define SF_K_COUNT 4
 int sf_time_steps_detailed(sf_ctx* ctx, int nsteps, double* total_ms, double* kernel_ms, int* launches);
 
+/* ---- batched ensemble ----------------------------------------------------
+ * The reference advances one solver per process (the StateVector::solver
+ * singleton, StateVector.h:546); an ensemble of independent runs is a loop of
+ * FullEvolve/FixedEvolve calls, each TimeStep (IMEXRK.cpp:8-24) on its own.
+ * A batch advances many independent 2D contexts (any N1, N3, Re, Ri, Pr, dt)
+ * together: runs with the same N1 share each x-pass launch and runs with the
+ * same vertical tile share each modal launch, and one step of all runs is a
+ * HIP graph replay.  Each context keeps its own state, set up through the
+ * usual calls (sf_set_background, uploads, sf_filter_all, sf_populate_nodal,
+ * sf_remove_divergence, sf_set_dt); sf_batch_step(b, n) then has the effect of
+ * sf_step(ctx, n) on every member.  The member contexts must outlive the
+ * batch, and all members must have the same precision and be 2D. */
+typedef struct sf_batch sf_batch;
+int sf_batch_create(sf_ctx* const* runs, int nruns, sf_batch** out);
+int sf_batch_destroy(sf_batch* b);
+const char* sf_batch_last_error(const sf_batch* b);
+int sf_batch_step(sf_batch* b, int nsteps);
+/* as sf_batch_step, returning the GPU time of the graph replays */
+int sf_batch_time_steps(sf_batch* b, int nsteps, double* total_ms);
+int sf_batch_info(const sf_batch* b, int* launches_per_step, int* xpass_groups, int* modal_groups);
+
 #ifdef __cplusplus
 }
 #endif

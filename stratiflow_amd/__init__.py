@@ -57,6 +57,8 @@ EXPORTED = [
     "sf_ddx", "sf_ddy", "sf_ddz", "sf_inner_prod", "sf_state_field", "sf_step", "sf_step_linear",
     "sf_set_linear_background", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
     "sf_energy", "sf_cfl", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
+    "sf_batch_create", "sf_batch_destroy", "sf_batch_last_error", "sf_batch_step", "sf_batch_time_steps",
+    "sf_batch_info",
 ]
 KERNEL_CLASSES = ["yinv", "xpass", "yfwd", "modal"]
 
@@ -111,6 +113,12 @@ def lib():
         "sf_synchronize": [vp],
         "sf_time_steps": [vp, i, dp],
         "sf_time_steps_detailed": [vp, i, dp, dp, ctypes.POINTER(ctypes.c_int)],
+        "sf_batch_create": [vpp, i, vpp],
+        "sf_batch_destroy": [vp],
+        "sf_batch_step": [vp, i],
+        "sf_batch_time_steps": [vp, i, dp],
+        "sf_batch_info": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                          ctypes.POINTER(ctypes.c_int)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -118,6 +126,8 @@ def lib():
         f.restype = ctypes.c_int
     L.sf_last_error.argtypes = [vp]
     L.sf_last_error.restype = ctypes.c_char_p
+    L.sf_batch_last_error.argtypes = [vp]
+    L.sf_batch_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -349,3 +359,49 @@ class IMEXRK:
         c = (ctypes.c_int * len(KERNEL_CLASSES))()
         self._chk(lib().sf_time_steps_detailed(self.h, int(n), ctypes.byref(ms), k, c))
         return ms.value, {name: (k[i], c[i]) for i, name in enumerate(KERNEL_CLASSES)}
+
+
+class Batch:
+    """Many independent 2D solvers advanced together (sf_batch_*).
+
+    ``runs`` are IMEXRK objects of one precision, already set up (background,
+    state, projection, dt); ``step(n)`` has the effect of ``run.TimeStep(n)``
+    on every run.  The runs must stay alive while the batch exists."""
+
+    def __init__(self, runs):
+        L = lib()
+        self.runs = list(runs)
+        arr = (ctypes.c_void_p * len(self.runs))(*[r.h.value for r in self.runs])
+        h = ctypes.c_void_p()
+        rc = L.sf_batch_create(arr, len(self.runs), ctypes.byref(h))
+        if rc != 0:
+            raise SFError(rc, L.sf_batch_last_error(None).decode())
+        self.h = h
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SFError(rc, lib().sf_batch_last_error(self.h).decode())
+
+    def step(self, n=1):
+        self._chk(lib().sf_batch_step(self.h, n))
+
+    def time_steps(self, n):
+        ms = ctypes.c_double()
+        self._chk(lib().sf_batch_time_steps(self.h, n, ctypes.byref(ms)))
+        return ms.value
+
+    def info(self):
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._chk(lib().sf_batch_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"launches_per_step": a.value, "xpass_groups": b.value, "modal_groups": c.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,6 +1,7 @@
 // sf_api.hip -- context, fields and the extern "C" boundary (include/sfhip.h).
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -88,6 +89,13 @@ struct sf_ctx {
     int precision;
     std::string err;
     std::shared_ptr<void> impl;  // sf::Ctx<T>
+    void* raw;
+};
+
+struct sf_batch {
+    int precision;
+    std::string err;
+    std::shared_ptr<void> impl;  // sf::Batch<T>
     void* raw;
 };
 
@@ -441,34 +449,75 @@ struct Ctx {
             check(launch_yfwd<T>(f, N2, stream), "yfwd");
             mark(SF_K_YFWD);
         } else {
-            XArgs<T> x{};
-            x.N3 = N3;
-            x.N3p = N3p;
-            x.A1 = A1;
-            x.zch = zch;
-            x.lin = modal_lay();
-            x.lout = modal_lay();
-            x.in[0] = src(0);
-            x.in[1] = nullptr;
-            x.in[2] = src(2);
-            x.in[3] = src(3);
-            x.out[0] = nl[0];
-            x.out[1] = nl[1];
-            x.out[2] = nl[2];
-            x.out[3] = nl[3];
-            for (int i = 4; i < 8; ++i) x.out[i] = nullptr;
-            x.zc = zc;
-            x.kx = kx;
-            x.tw = tw1;
-            x.scale = T(1.0 / double(N1));
-            x.linear = linear ? 1 : 0;
-            x.base[0] = mf(SF_U1_TOT);
-            x.base[1] = nullptr;
-            x.base[2] = mf(SF_U3_TOT);
-            x.base[3] = mf(SF_B_TOT);
+            const XArgs<T> x = xargs_2d(linear);
             check(launch_xpass<T>(x, N1, 1, false, stream), "xpass");
             mark(SF_K_XPASS);
         }
+    }
+
+    // arguments of the 2D x-pass (IMEXRK.cpp:136-193, or :195-258 when linear);
+    // from_state: read the prognostic fields even while the populated nodal
+    // snapshot is pending (for arguments reused by every later step)
+    XArgs<T> xargs_2d(bool linear, bool from_state = false) {
+        auto src = [&](int i) -> const V* { return from_state ? mf(SF_U1 + i) : this->src(i); };
+        XArgs<T> x{};
+        x.N3 = N3;
+        x.N3p = N3p;
+        x.A1 = A1;
+        x.zch = zch;
+        x.lin = modal_lay();
+        x.lout = modal_lay();
+        x.in[0] = src(0);
+        x.in[1] = nullptr;
+        x.in[2] = src(2);
+        x.in[3] = src(3);
+        x.out[0] = nl[0];
+        x.out[1] = nl[1];
+        x.out[2] = nl[2];
+        x.out[3] = nl[3];
+        for (int i = 4; i < 8; ++i) x.out[i] = nullptr;
+        x.zc = zc;
+        x.kx = kx;
+        x.tw = tw1;
+        x.scale = T(1.0 / double(N1));
+        x.linear = linear ? 1 : 0;
+        x.base[0] = mf(SF_U1_TOT);
+        x.base[1] = nullptr;
+        x.base[2] = mf(SF_U3_TOT);
+        x.base[3] = mf(SF_B_TOT);
+        return x;
+    }
+
+    // arguments of the modal kernel for stage k with the explicit terms of the
+    // previous stage in rbuf[rc] (IMEXRK.cpp:8-24; ExplicitRK :119-134,
+    // FinishRHS :107-117, CrankNicolson :75-105, RemoveDivergence :43-73)
+    ModalArgs<T> stage_modal_args(int k, bool store_r, int rc) {
+        static const double beta[3] = {1.0, 25.0 / 8.0, 9.0 / 4.0};    // IMEXRK.h:631
+        static const double zeta[3] = {0.0, -17.0 / 8.0, -5.0 / 4.0};  // IMEXRK.h:632
+        ModalArgs<T> m = modal_args();
+        const int nxt = 1 - rc;
+        const bool read_old = (k > 0);
+        m.ro1 = read_old ? rbuf[rc][0] : nullptr;
+        m.ro2 = (read_old && three) ? rbuf[rc][1] : nullptr;
+        m.ro3 = read_old ? rbuf[rc][2] : nullptr;
+        m.roB = read_old ? rbuf[rc][3] : nullptr;
+        m.rn1 = store_r ? rbuf[nxt][0] : nullptr;
+        m.rn2 = (store_r && three) ? rbuf[nxt][1] : nullptr;
+        m.rn3 = store_r ? rbuf[nxt][2] : nullptr;
+        m.rnB = store_r ? rbuf[nxt][3] : nullptr;
+        m.n1 = nl[0];
+        m.n2 = three ? nl[1] : nullptr;
+        m.n3 = nl[2];
+        m.nB = nl[3];
+        m.hz = h[k] * T(zeta[k]);
+        m.mh = -h[k];
+        m.hb = h[k] * T(beta[k]);
+        m.cv = (T(0.5) * h[k]) / T(Re);
+        m.cb = ((T(0.5) * h[k]) / T(Re)) / T(Pr);
+        m.Ri = T(Ri);
+        m.pmult = T(1) / h[k];
+        m.mode = 0;
+        return m;
     }
 
     ModalArgs<T> modal_args() {
@@ -494,32 +543,9 @@ struct Ctx {
 
     // one stage k; IMEXRK.cpp:8-24 (and :26-41 when linear)
     void stage(int k, bool linear, bool store_r) {
-        static const double beta[3] = {1.0, 25.0 / 8.0, 9.0 / 4.0};    // IMEXRK.h:631
-        static const double zeta[3] = {0.0, -17.0 / 8.0, -5.0 / 4.0};  // IMEXRK.h:632
         transform_path(linear);
-        ModalArgs<T> m = modal_args();
+        ModalArgs<T> m = stage_modal_args(k, store_r, rcur);
         const int nxt = 1 - rcur;
-        const bool read_old = (k > 0);
-        m.ro1 = read_old ? rbuf[rcur][0] : nullptr;
-        m.ro2 = (read_old && three) ? rbuf[rcur][1] : nullptr;
-        m.ro3 = read_old ? rbuf[rcur][2] : nullptr;
-        m.roB = read_old ? rbuf[rcur][3] : nullptr;
-        m.rn1 = store_r ? rbuf[nxt][0] : nullptr;
-        m.rn2 = (store_r && three) ? rbuf[nxt][1] : nullptr;
-        m.rn3 = store_r ? rbuf[nxt][2] : nullptr;
-        m.rnB = store_r ? rbuf[nxt][3] : nullptr;
-        m.n1 = nl[0];
-        m.n2 = three ? nl[1] : nullptr;
-        m.n3 = nl[2];
-        m.nB = nl[3];
-        m.hz = h[k] * T(zeta[k]);
-        m.mh = -h[k];
-        m.hb = h[k] * T(beta[k]);
-        m.cv = (T(0.5) * h[k]) / T(Re);
-        m.cb = ((T(0.5) * h[k]) / T(Re)) / T(Pr);
-        m.Ri = T(Ri);
-        m.pmult = T(1) / h[k];
-        m.mode = 0;
         check(launch_modal<T>(m, stream), "modal");
         mark(SF_K_MODAL);
         use_snap = false;
@@ -731,6 +757,242 @@ struct Ctx {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Batched ensemble: many independent 2D runs (each its own context, grid and
+// parameters) advanced together.  Runs with the same N1 share one x-pass
+// launch and runs with the same rows-per-lane share one modal launch; both
+// kernels loop over the work items of all their runs (grid-stride, so a fixed
+// number of resident blocks drains every run).  One step is 3 x (x-pass
+// groups + modal groups) launches, captured once in a HIP graph and replayed.
+// The levels per x-pass block and the number of resident blocks of each
+// x-pass group are picked by timing the candidates when the batch is built.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Batch {
+    struct Group {
+        int key = 0;  // N1 (x-pass) or rows per lane M (modal)
+        std::vector<int> members;
+        int total = 0;
+        int grid = 1;
+        int zch = 2;
+        void* args = nullptr;
+        int* start = nullptr;
+    };
+    std::vector<Ctx<T>*> runs;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<Group> xg;
+    std::vector<Group> mg[3];
+    std::vector<double> dts;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int ncu = 256;
+    int launches_per_step = 0;
+
+    ~Batch() {
+        release_graph();
+        for (auto& g : xg) free_group(g);
+        for (auto& v : mg)
+            for (auto& g : v) free_group(g);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    static void free_group(Group& g) {
+        if (g.args) (void)hipFree(g.args);
+        if (g.start) (void)hipFree(g.start);
+        g.args = nullptr;
+        g.start = nullptr;
+    }
+    void release_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+
+    void init(const std::vector<Ctx<T>*>& r) {
+        runs = r;
+        SF_REQUIRE(!runs.empty(), SF_ERR_ARG, "empty batch");
+        for (auto* c : runs) SF_REQUIRE(!c->three, SF_ERR_UNSUPPORTED, "batched runs must be 2D (N2 == 1)");
+        int dev = 0;
+        SF_HIP(hipGetDevice(&dev));
+        SF_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        SF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        SF_HIP(hipEventCreate(&ev0));
+        SF_HIP(hipEventCreate(&ev1));
+        // groups
+        std::map<int, std::vector<int>> byN1, byM;
+        for (int i = 0; i < int(runs.size()); ++i) {
+            byN1[runs[i]->N1].push_back(i);
+            byM[(runs[i]->N3 + 63) / 64].push_back(i);
+        }
+        for (auto& kv : byN1) {
+            Group g;
+            g.key = kv.first;
+            g.members = kv.second;
+            xg.push_back(g);
+        }
+        for (int k = 0; k < 3; ++k)
+            for (auto& kv : byM) {
+                Group g;
+                g.key = kv.first;
+                g.members = kv.second;
+                mg[k].push_back(g);
+            }
+        sync_runs();
+        tune();
+        build();
+    }
+
+    void sync_runs() {
+        for (auto* c : runs) SF_HIP(hipStreamSynchronize(c->stream));
+    }
+
+    template <typename A>
+    void upload_group(Group& g, const std::vector<A>& a, const std::vector<int>& st) {
+        free_group(g);
+        SF_HIP(hipMalloc(&g.args, a.size() * sizeof(A)));
+        SF_HIP(hipMalloc(reinterpret_cast<void**>(&g.start), st.size() * sizeof(int)));
+        SF_HIP(hipMemcpy(g.args, a.data(), a.size() * sizeof(A), hipMemcpyHostToDevice));
+        SF_HIP(hipMemcpy(g.start, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+
+    void build_xgroup(Group& g, int zch, int grid) {
+        std::vector<XArgs<T>> a;
+        std::vector<int> st;
+        int total = 0;
+        for (int r : g.members) {
+            XArgs<T> x = runs[r]->xargs_2d(false, true);
+            x.zch = zch;
+            a.push_back(x);
+            st.push_back(total);
+            total += (runs[r]->N3 + zch - 1) / zch;
+        }
+        st.push_back(total);
+        g.zch = zch;
+        g.total = total;
+        g.grid = std::max(1, std::min(total, grid));
+        upload_group(g, a, st);
+    }
+
+    void build() {
+        release_graph();
+        for (auto& g : xg) build_xgroup(g, g.zch, g.grid);
+        for (int k = 0; k < 3; ++k)
+            for (auto& g : mg[k]) {
+                std::vector<ModalArgs<T>> a;
+                std::vector<int> st;
+                int total = 0;
+                for (int r : g.members) {
+                    Ctx<T>* c = runs[r];
+                    a.push_back(c->stage_modal_args(k, k < 2, c->rcur));
+                    st.push_back(total);
+                    total += (c->A1 * c->A2 + BS_MODES - 1) / BS_MODES;
+                }
+                st.push_back(total);
+                g.total = total;
+                g.grid = std::max(1, std::min(total, 8 * ncu));
+                upload_group(g, a, st);
+            }
+        dts.clear();
+        for (auto* c : runs) dts.push_back(c->dt);
+        launches_per_step = 3 * int(xg.size() + mg[0].size());
+    }
+
+    void launch_stage(int k) {
+        for (auto& g : xg) {
+            const hipError_t e = launch_xpass_batch<T>(static_cast<const XArgs<T>*>(g.args), g.start,
+                                                       int(g.members.size()), g.total, g.key, g.grid, stream);
+            if (e != hipSuccess) throw Error(SF_ERR_HIP, std::string("batched xpass: ") + hipGetErrorString(e));
+        }
+        for (auto& g : mg[k]) {
+            const hipError_t e = launch_modal_batch<T>(static_cast<const ModalArgs<T>*>(g.args), g.start,
+                                                       int(g.members.size()), g.total, g.key, g.grid, stream);
+            if (e != hipSuccess) throw Error(SF_ERR_HIP, std::string("batched modal: ") + hipGetErrorString(e));
+        }
+    }
+
+    // levels per block and resident blocks of each x-pass group, by timing
+    // (the x-pass only reads the state and rewrites the nonlinear terms)
+    void tune() {
+        for (auto& g : xg) {
+            double best = 1e30;
+            int bz = 2, bgrid = 8 * ncu;
+            for (int zch : {2, 4, 8, 16}) {
+                for (int mult : {4, 8, 16, 0}) {
+                    const int grid = mult ? mult * ncu : (1 << 30);
+                    build_xgroup(g, zch, grid);
+                    auto once = [&]() {
+                        const hipError_t e = launch_xpass_batch<T>(static_cast<const XArgs<T>*>(g.args), g.start,
+                                                                   int(g.members.size()), g.total, g.key, g.grid,
+                                                                   stream);
+                        if (e != hipSuccess) throw Error(SF_ERR_HIP, "batched xpass (tuning)");
+                    };
+                    once();
+                    SF_HIP(hipEventRecord(ev0, stream));
+                    for (int i = 0; i < 3; ++i) once();
+                    SF_HIP(hipEventRecord(ev1, stream));
+                    SF_HIP(hipEventSynchronize(ev1));
+                    float ms = 0;
+                    SF_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+                    if (ms < best) {
+                        best = ms;
+                        bz = zch;
+                        bgrid = grid;
+                    }
+                }
+            }
+            g.zch = bz;
+            g.grid = bgrid;
+        }
+    }
+
+    void capture() {
+        release_graph();
+        SF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int k = 0; k < 3; ++k) launch_stage(k);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        SF_HIP(hipStreamEndCapture(stream, &graph));
+        SF_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+
+    // advance every run by n TimeSteps (IMEXRK.cpp:8-24)
+    void step(int n, double* ms_out = nullptr) {
+        SF_REQUIRE(n >= 0, SF_ERR_ARG, "negative step count");
+        sync_runs();
+        bool stale = false;
+        for (size_t i = 0; i < runs.size(); ++i) stale = stale || (runs[i]->dt != dts[i]);
+        if (stale) build();
+        int first = 0;
+        bool snap = false;
+        for (auto* c : runs) snap = snap || c->use_snap;
+        if (snap && n > 0) {
+            // the first step after PopulateNodalVariables reads the populated
+            // nodal snapshot (StateVector.cpp:9-11): take it run by run
+            for (auto* c : runs) c->step(1, false);
+            sync_runs();
+            first = 1;
+        }
+        if (!exec) capture();
+        SF_HIP(hipEventRecord(ev0, stream));
+        for (int i = first; i < n; ++i) SF_HIP(hipGraphLaunch(exec, stream));
+        SF_HIP(hipEventRecord(ev1, stream));
+        SF_HIP(hipEventSynchronize(ev1));
+        if (ms_out) {
+            float ms = 0;
+            SF_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+            *ms_out = ms;
+        }
+    }
+};
+
 template <typename F>
 static int guarded(sf_ctx* c, F fn) {
     try {
@@ -764,6 +1026,22 @@ using namespace sf;
     do {                                                               \
         if (!(p)) return SF_ERR_ARG;                                   \
     } while (0)
+
+template <typename Fn>
+static int batch_call(sf_batch* b, Fn fn) {
+    try {
+        if (b->precision == 4) fn(*static_cast<Batch<float>*>(b->raw));
+        else fn(*static_cast<Batch<double>*>(b->raw));
+        b->err.clear();
+        return SF_OK;
+    } catch (const Error& e) {
+        b->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        b->err = e.what();
+        return SF_ERR_STATE;
+    }
+}
 
 extern "C" {
 
@@ -1152,6 +1430,66 @@ int sf_time_steps_detailed(sf_ctx* ctx, int nsteps, double* total_ms, double* ke
                 }
             }
         });
+    });
+}
+
+// ---- batched ensemble ------------------------------------------------------
+int sf_batch_create(sf_ctx* const* runs, int nruns, sf_batch** out) {
+    SF_NULLCHK(runs);
+    SF_NULLCHK(out);
+    *out = nullptr;
+    if (nruns <= 0) return SF_ERR_ARG;
+    const int prec = runs[0] ? runs[0]->precision : 0;
+    for (int i = 0; i < nruns; ++i)
+        if (!runs[i] || runs[i]->precision != prec) return SF_ERR_ARG;
+    std::unique_ptr<sf_batch> b(new sf_batch());
+    b->precision = prec;
+    const int rc = guarded(nullptr, [&]() {
+        if (prec == 4) {
+            auto impl = std::make_shared<Batch<float>>();
+            std::vector<Ctx<float>*> r;
+            for (int i = 0; i < nruns; ++i) r.push_back(static_cast<Ctx<float>*>(runs[i]->raw));
+            impl->init(r);
+            b->raw = impl.get();
+            b->impl = impl;
+        } else {
+            auto impl = std::make_shared<Batch<double>>();
+            std::vector<Ctx<double>*> r;
+            for (int i = 0; i < nruns; ++i) r.push_back(static_cast<Ctx<double>*>(runs[i]->raw));
+            impl->init(r);
+            b->raw = impl.get();
+            b->impl = impl;
+        }
+    });
+    if (rc != SF_OK) return rc;
+    *out = b.release();
+    return SF_OK;
+}
+
+int sf_batch_destroy(sf_batch* b) {
+    delete b;
+    return SF_OK;
+}
+
+const char* sf_batch_last_error(const sf_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int sf_batch_step(sf_batch* b, int nsteps) {
+    SF_NULLCHK(b);
+    return batch_call(b, [&](auto& B) { B.step(nsteps); });
+}
+
+int sf_batch_time_steps(sf_batch* b, int nsteps, double* total_ms) {
+    SF_NULLCHK(b);
+    SF_NULLCHK(total_ms);
+    return batch_call(b, [&](auto& B) { B.step(nsteps, total_ms); });
+}
+
+int sf_batch_info(const sf_batch* b, int* launches_per_step, int* xpass_groups, int* modal_groups) {
+    SF_NULLCHK(b);
+    return batch_call(const_cast<sf_batch*>(b), [&](auto& B) {
+        if (launches_per_step) *launches_per_step = B.launches_per_step;
+        if (xpass_groups) *xpass_groups = int(B.xg.size());
+        if (modal_groups) *modal_groups = int(B.mg[0].size());
     });
 }
 

@@ -4,6 +4,9 @@
 
 namespace sf {
 
+// modes per block of the modal kernels (one wavefront per mode)
+constexpr int BS_MODES = 4;
+
 // Fused per-mode stage kernel (one wavefront per Fourier mode).
 template <typename T>
 struct ModalArgs {
@@ -78,6 +81,15 @@ template <typename T>
 hipError_t launch_modal(const ModalArgs<T>& a, hipStream_t s);
 template <typename T>
 hipError_t launch_xpass(const XArgs<T>& a, int N1, int N2, bool threeD, hipStream_t s);
+// many independent 2D runs per launch: args/start live in device memory,
+// start[r] = first work item of run r (x-pass: z-blocks; modal: groups of
+// four modes), total = start[nruns]; grid-stride loop over `grid` blocks
+template <typename T>
+hipError_t launch_modal_batch(const ModalArgs<T>* args, const int* start, int nruns, int total, int M, int grid,
+                              hipStream_t s);
+template <typename T>
+hipError_t launch_xpass_batch(const XArgs<T>* args, const int* start, int nruns, int total, int N1, int grid,
+                              hipStream_t s);
 template <typename T>
 hipError_t launch_yinv(const YArgs<T>& a, int N2, hipStream_t s);
 template <typename T>

@@ -25,6 +25,7 @@
 namespace sf {
 
 constexpr int BS = 256;
+static_assert(BS / 64 == BS_MODES, "modal kernels: one wavefront per mode");
 
 // ---------------------------------------------------------------------------
 // k_modal
@@ -118,10 +119,9 @@ __device__ __forceinline__ void build_rows(int kind, const ModalArgs<T>& A, cons
 }
 
 template <typename T, int M>
-__global__ __launch_bounds__(BS) void k_modal(ModalArgs<T> A) {
+__device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int mode) {
     using V = cx<T>;
     const int lane = threadIdx.x & 63;
-    const int mode = blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
     if (mode >= A.nmodes) return;  // wave-uniform
     const int j1 = mode % A.A1;
     const int j2 = mode / A.A1;
@@ -333,6 +333,51 @@ __global__ __launch_bounds__(BS) void k_modal(ModalArgs<T> A) {
     store(A.p, pp);
 }
 
+template <typename T, int M>
+__global__ __launch_bounds__(BS) void k_modal(ModalArgs<T> A) {
+    modal_body<T, M>(A, blockIdx.x * (BS / 64) + (threadIdx.x >> 6));
+}
+
+// index r of the run owning work item `item`: start[r] <= item < start[r+1]
+__device__ __forceinline__ int find_run(const int* __restrict__ start, int nruns, int item) {
+    int lo = 0, hi = nruns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Many independent runs in one launch: a grid-stride (persistent) loop over
+// the work items of all runs, BS/64 modes per item.  Every run has the same
+// rows-per-lane M; each reads its own arguments.
+template <typename T, int M>
+__global__ __launch_bounds__(BS) void k_modal_batch(const ModalArgs<T>* __restrict__ args,
+                                                    const int* __restrict__ start, int nruns, int total) {
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int r = find_run(start, nruns, item);
+        modal_body<T, M>(args[r], (item - start[r]) * (BS / 64) + (threadIdx.x >> 6));
+    }
+}
+
+template <typename T>
+hipError_t launch_modal_batch(const ModalArgs<T>* args, const int* start, int nruns, int total, int M, int grid,
+                              hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    switch (M) {
+#define SF_M(m)                                                                                     \
+    case m:                                                                                         \
+        hipLaunchKernelGGL((k_modal_batch<T, m>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total); \
+        break;
+        SF_M(1) SF_M(2) SF_M(3) SF_M(4) SF_M(5) SF_M(6) SF_M(7) SF_M(8)
+#undef SF_M
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_modal(const ModalArgs<T>& a, hipStream_t s) {
     const int M = (a.N3 + 63) / 64;
@@ -378,7 +423,7 @@ bool tz_supported(int n) {
 }
 
 template <typename T, int N1, bool THREED, bool LIN>
-__global__ __launch_bounds__(BS) void k_xpass(XArgs<T> A) {
+__device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, const int n2) {
     using V = cx<T>;
     constexpr int LDP = fft_ldp<N1>();
     constexpr int NI = LIN ? 4 : 2;        // input pair lines per level
@@ -391,9 +436,8 @@ __global__ __launch_bounds__(BS) void k_xpass(XArgs<T> A) {
     V* const prd = lds;
     V* const ring = lds + NP * LDP;
 
-    const int n2 = blockIdx.y;
     const int N3 = A.N3, A1 = A.A1;
-    const int zbeg = blockIdx.x * A.zch;
+    const int zbeg = bx * A.zch;
     const int zend = min(zbeg + A.zch, N3);
     const int tid = threadIdx.x;
     auto slot = [&](int z) { return (z + 1 - zbeg) % 3; };
@@ -594,6 +638,38 @@ __global__ __launch_bounds__(BS) void k_xpass(XArgs<T> A) {
             }
         }
         sync_lds();
+    }
+}
+
+template <typename T, int N1, bool THREED, bool LIN>
+__global__ __launch_bounds__(BS) void k_xpass(XArgs<T> A) {
+    xpass_body<T, N1, THREED, LIN>(A, blockIdx.x, blockIdx.y);
+}
+
+// 2D runs with the same N1 in one launch (grid-stride over the z-blocks of
+// all runs; see k_modal_batch)
+template <typename T, int N1>
+__global__ __launch_bounds__(BS) void k_xpass_batch(const XArgs<T>* __restrict__ args, const int* __restrict__ start,
+                                                    int nruns, int total) {
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int r = find_run(start, nruns, item);
+        xpass_body<T, N1, false, false>(args[r], item - start[r], 0);
+    }
+}
+
+template <typename T>
+hipError_t launch_xpass_batch(const XArgs<T>* args, const int* start, int nruns, int total, int N1, int grid,
+                              hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    switch (N1) {
+#define SF_CASE(n)                                                                                        \
+    case n:                                                                                               \
+        hipLaunchKernelGGL((k_xpass_batch<T, n>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total); \
+        return hipGetLastError();
+        SF_SIZES(SF_CASE)
+#undef SF_CASE
+        default:
+            return hipErrorInvalidValue;
     }
 }
 
@@ -1104,6 +1180,8 @@ hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, cons
 
 #define SF_INST(T)                                                                                                  \
     template hipError_t launch_modal<T>(const ModalArgs<T>&, hipStream_t);                                           \
+    template hipError_t launch_modal_batch<T>(const ModalArgs<T>*, const int*, int, int, int, int, hipStream_t);      \
+    template hipError_t launch_xpass_batch<T>(const XArgs<T>*, const int*, int, int, int, int, hipStream_t);          \
     template hipError_t launch_xpass<T>(const XArgs<T>&, int, int, bool, hipStream_t);                               \
     template hipError_t launch_yinv<T>(const YArgs<T>&, int, hipStream_t);                                           \
     template hipError_t launch_yfwd<T>(const YArgs<T>&, int, hipStream_t);                                           \
