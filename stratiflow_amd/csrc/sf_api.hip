@@ -886,7 +886,11 @@ struct Batch {
                 int total = 0;
                 for (int r : g.members) {
                     Ctx<T>* c = runs[r];
-                    a.push_back(c->stage_modal_args(k, k < 2, c->rcur));
+                    // the explicit terms ping-pong as in Ctx::stage: stage 0
+                    // writes rbuf[1-rcur], stage 1 reads it and writes rbuf[rcur],
+                    // stage 2 reads rbuf[rcur]; after a step rcur is unchanged
+                    const int rc = (k == 1) ? 1 - c->rcur : c->rcur;
+                    a.push_back(c->stage_modal_args(k, k < 2, rc));
                     st.push_back(total);
                     total += (c->A1 * c->A2 + BS_MODES - 1) / BS_MODES;
                 }

@@ -17,7 +17,7 @@
 // F(x) = Phi(x) - x with Phi = nsteps fixed-dt steps (FixedEvolve start-up)
 class FixedNewton : public NewtonKrylov<StateVector> {
 public:
-    stratifloat dt = 0.02f;
+    stratifloat dt = stratifloat(0.02);
     int nsteps = 25;
 
 protected:
@@ -64,7 +64,11 @@ int main(int argc, char** argv) {
     SeedRandom(7);
     StateVector x;
     x.ExciteLowWavenumbers(stratifloat(1e-3));
+    // through the nodal .fields file and back, so that the state the harness
+    // reads is exactly this one (a .fields file cannot hold the imaginary part
+    // of the j1 = 0 coefficients that the random excitation draws)
     x.SaveToFile(dir + "/x0");
+    x.LoadFromFile(dir + "/x0.fields");
 
     FixedNewton nk;
     nk.verbose = false;
@@ -85,6 +89,7 @@ int main(int argc, char** argv) {
     StateVector q0;
     q0.ExciteLowWavenumbers(stratifloat(0.1));
     q0.SaveToFile(dir + "/q0");
+    q0.LoadFromFile(dir + "/q0.fields");
     LinearArnoldi ar(stratifloat(0.5));
     ar.K = K + 4;
     ar.startVector = &q0;
