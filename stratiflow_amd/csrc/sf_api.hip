@@ -109,7 +109,6 @@ struct Ctx {
     sf_ctx* owner = nullptr;
     int N1 = 0, N2 = 0, N3 = 0, M1 = 0, A1 = 0, A2 = 0, dim = 0, N3p = 0;
     bool three = false;
-    int zt = 4;    // z tile of the x-spectral intermediates
     int zch = 4;   // z levels per x-pass block
     double L1 = 0, L2 = 0, L3 = 0, Re = 0, Ri = 0, Pr = 0;
     double dt = 0.01;
@@ -203,16 +202,6 @@ struct Ctx {
         A1 = N1 / 3;
         A2 = three ? 2 * (N2 / 3) - 1 : 1;
         N3p = N3;
-        // z tile of the intermediates: largest of 16, 8, 4 dividing N3 whose
-        // paired forward y pass (2*zt lines) fits the y kernels' LDS budget
-        const int prec = int(sizeof(T));
-        zt = 4;
-        for (int c : {16, 8}) {
-            if (N3 % c == 0 && (!three || 2 * c <= ypass_max_lines(N2, prec))) {
-                zt = c;
-                break;
-            }
-        }
         zch = xpass_levels(N2, N3);
         nmodal = size_t(A1) * A2 * N3;
         nnodal = size_t(N1) * N2 * N3;
@@ -382,11 +371,6 @@ struct Ctx {
 
     const V* src(int i) { return use_snap ? snap[i] : mf(SF_U1 + i); }
 
-    Lay modal_lay() const { return Lay{N3, -1, 0, 0, size_t(N3)}; }
-    Lay tiled_lay() const {
-        const int sh = zt == 16 ? 4 : (zt == 8 ? 3 : 2);
-        return Lay{zt, sh, size_t(A1) * N3, size_t(A1) * zt, size_t(zt)};
-    }
 
     // nonlinear (or linearised) explicit terms of the current state into nl[]
     void transform_path(bool linear) {
@@ -396,7 +380,6 @@ struct Ctx {
             a.A1 = A1;
             a.A2 = A2;
             a.nf = 4;
-            a.lay = tiled_lay();
             for (int i = 0; i < 4; ++i) a.in[i] = src(i);
             for (int i = 0; i < 4; ++i) a.out[i] = Y[i];
             a.tw = tw2;
@@ -419,8 +402,6 @@ struct Ctx {
             x.N3p = N3p;
             x.A1 = A1;
             x.zch = zch;
-            x.lin = tiled_lay();
-            x.lout = tiled_lay();
             for (int i = 0; i < 4; ++i) x.in[i] = Y[i];
             for (int i = 0; i < 8; ++i) x.out[i] = X[i];
             x.zc = zc;
@@ -437,7 +418,6 @@ struct Ctx {
             f.A2 = A2;
             f.nf = 4;
             f.paired = 1;
-            f.lay = tiled_lay();
             for (int r = 0; r < 4; ++r) {
                 f.in[2 * r] = X[r];
                 f.in[2 * r + 1] = X[4 + r];
@@ -465,8 +445,6 @@ struct Ctx {
         x.N3p = N3p;
         x.A1 = A1;
         x.zch = zch;
-        x.lin = modal_lay();
-        x.lout = modal_lay();
         x.in[0] = src(0);
         x.in[1] = nullptr;
         x.in[2] = src(2);
@@ -582,17 +560,14 @@ struct Ctx {
             a.A1 = A1;
             a.A2 = A2;
             a.nf = 1;
-            a.lay = tiled_lay();
             a.in[0] = modal;
             a.out[0] = X[0];
             a.tw = tw2;
             a.scale = T(1);
             check(launch_yinv<T>(a, N2, stream), "to_nodal:y");
             x.spec_in = X[0];
-            x.lay = tiled_lay();
         } else {
             x.spec_in = modal;
-            x.lay = modal_lay();
         }
         check(launch_x_c2r<T>(x, N1, N2, stream), "to_nodal:x");
     }
@@ -607,7 +582,6 @@ struct Ctx {
         x.nzt = 16;
         if (three) {
             x.spec_out = X[0];
-            x.lay = tiled_lay();
             x.scale = T(1);
             check(launch_x_r2c<T>(x, N1, N2, stream), "to_modal:x");
             YArgs<T> a{};
@@ -616,7 +590,6 @@ struct Ctx {
             a.A2 = A2;
             a.nf = 1;
             a.paired = 0;
-            a.lay = tiled_lay();
             a.in[0] = X[0];
             a.out[0] = modal;
             a.tw = tw2;
@@ -625,7 +598,6 @@ struct Ctx {
             check(launch_yfwd<T>(a, N2, stream), "to_modal:y");
         } else {
             x.spec_out = modal;
-            x.lay = modal_lay();
             x.scale = T(1.0 / double(N1));
             check(launch_x_r2c<T>(x, N1, N2, stream), "to_modal:x");
         }
@@ -647,7 +619,7 @@ struct Ctx {
     // dot products into scal[slot]
     void dot_async(const V* a, const V* b, int bc, int slot) {
         const T* w = zc + size_t(bc == SF_NEUMANN ? ZC_WN : ZC_WD) * N3p;
-        check(launch_dot<T>(a, b, A1 * A2, N3, w, partial, scal, slot, stream), "dot");
+        check(launch_dot<T>(a, b, A1, A2, N3, w, partial, scal, slot, stream), "dot");
     }
     void read_scalars(int n, double* out) {
         SF_HIP(hipMemcpyAsync(scal_host, scal, n * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -701,7 +673,7 @@ struct Ctx {
                 for (int j1 = 0; j1 < A1; ++j1)
                     for (int j3 = 0; j3 < N3; ++j3) {
                         const size_t src = (size_t(M1) * j2 + j1) * N3 + j3;
-                        const size_t dst = (size_t(A1) * ja + j1) * N3 + j3;
+                        const size_t dst = sidx(ja, j3, j1, N3, A1);
                         h[dst].x = T(host[2 * src]);
                         h[dst].y = T(host[2 * src + 1]);
                     }
@@ -727,7 +699,7 @@ struct Ctx {
                 for (int j1 = 0; j1 < A1; ++j1)
                     for (int j3 = 0; j3 < N3; ++j3) {
                         const size_t dst = (size_t(M1) * j2 + j1) * N3 + j3;
-                        const size_t src = (size_t(A1) * ja + j1) * N3 + j3;
+                        const size_t src = sidx(ja, j3, j1, N3, A1);
                         host[2 * dst] = h[src].x;
                         host[2 * dst + 1] = h[src].y;
                     }
@@ -892,7 +864,8 @@ struct Batch {
                     const int rc = (k == 1) ? 1 - c->rcur : c->rcur;
                     a.push_back(c->stage_modal_args(k, k < 2, rc));
                     st.push_back(total);
-                    total += (c->A1 * c->A2 + BS_MODES - 1) / BS_MODES;
+                    const int mb = modal_modes_per_block(g.key, int(sizeof(T)));
+                    total += ((c->A1 + mb - 1) / mb) * c->A2;
                 }
                 st.push_back(total);
                 g.total = total;
@@ -1214,7 +1187,7 @@ static int ends_op(sf_field* f, int op) {
         SF_REQUIRE(f->kind == SF_MODAL, SF_ERR_UNSUPPORTED, "end conditions are implemented for modal fields");
         with_ctx(f->ctx, [&](auto& C) {
             using T = typename std::remove_reference<decltype(C.h[0])>::type;
-            C.check(launch_ends<T>(static_cast<cx<T>*>(f->d), C.A1 * C.A2, C.N3, op, C.stream), "ends");
+            C.check(launch_ends<T>(static_cast<cx<T>*>(f->d), C.A1, C.A2, C.N3, op, C.stream), "ends");
         });
     });
 }

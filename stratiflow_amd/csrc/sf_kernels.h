@@ -4,8 +4,6 @@
 
 namespace sf {
 
-// modes per block of the modal kernels (one wavefront per mode)
-constexpr int BS_MODES = 4;
 
 // Fused per-mode stage kernel (one wavefront per Fourier mode).
 template <typename T>
@@ -21,26 +19,20 @@ struct ModalArgs {
     int mode;  // 0: full stage, 1: projection only (RemoveDivergence)
 };
 
-// Position of element (n2, j1, z) of an x-spectral field:
-//   n2*s_n2 + (z / zt)*s_zb + j1*s_j1 + z % zt
-// 3D intermediates are z-tiled ([n2][z/zt][j1][z%zt], zt = 16 when N3 allows)
-// so that an x-pass block reads and writes whole contiguous tiles; modal
-// fields are the special case zt = N3, s_j1 = N3 ([j1][z]).
-struct Lay {
-    int zt;   // tile height (a power of two), or N3 for the untiled modal layout
-    int zsh;  // log2(zt), or -1 when untiled
-    size_t s_n2, s_zb, s_j1;
-};
-__host__ __device__ inline size_t lay_at(const Lay& L, int n2, int j1, int z) {
-    if (L.zsh < 0) return size_t(n2) * L.s_n2 + size_t(j1) * L.s_j1 + size_t(z);
-    return size_t(n2) * L.s_n2 + size_t(z >> L.zsh) * L.s_zb + size_t(j1) * L.s_j1 + size_t(z & (L.zt - 1));
+// Position of element (row, z, j1) of every spectral array: row is the active
+// y-mode ja of a modal field ([ja][z][j1]) or the spanwise point n2 of an
+// x-spectral intermediate ([n2][z][j1]).  j1 is fastest, so the x transforms
+// read and write whole contiguous rows of A1 values, the y transforms read and
+// write runs of a j1 tile, and the modal kernels stage a tile of consecutive
+// j1 modes through LDS.
+__host__ __device__ inline size_t sidx(int row, int z, int j1, int N3, int A1) {
+    return (size_t(row) * size_t(N3) + size_t(z)) * size_t(A1) + size_t(j1);
 }
 
 // Fused inverse-x / products / forward-x kernel.
 template <typename T>
 struct XArgs {
-    int N3, N3p, A1, zch;                // zch: z levels per block (a multiple of the sub-tile)
-    Lay lin, lout;
+    int N3, N3p, A1, zch;                // zch: z levels per block
     const cx<T>* in[4];                  // u1, u2, u3, b spectra
     cx<T>* out[8];                       // 3D: A1,A2,A3,AB,B1,B2,B3,BB ; 2D: Nl1,-,Nl3,NlB
     const T* zc;
@@ -48,14 +40,13 @@ struct XArgs {
     const cx<T>* tw;
     T scale;
     int linear;                          // 1: linearised products (BuildRHSLinear)
-    const cx<T>* base[4];                // linear: u1t, u2t, u3t, bt spectra (layout lin)
+    const cx<T>* base[4];                // linear: u1t, u2t, u3t, bt spectra (same layout as in)
 };
 
-// y-direction transforms (3D): per (j1, z-tile of lay.zt levels).
+// y-direction transforms (3D): per (z, tile of consecutive j1).
 template <typename T>
 struct YArgs {
     int N3, A1, A2, nf;
-    Lay lay;          // layout of the x-spectral side
     const cx<T>* in[8];
     cx<T>* out[8];
     const T* ky;      // A2 active wavenumbers (forward: i*ky*B term)
@@ -68,7 +59,6 @@ struct YArgs {
 template <typename T>
 struct XTArgs {
     int N3, A1, nzt;
-    Lay lay;
     const cx<T>* spec_in;
     cx<T>* spec_out;
     const T* nodal_in;
@@ -99,7 +89,7 @@ hipError_t launch_x_c2r(const XTArgs<T>& a, int N1, int N2, hipStream_t s);
 template <typename T>
 hipError_t launch_x_r2c(const XTArgs<T>& a, int N1, int N2, hipStream_t s);
 template <typename T>
-hipError_t launch_dot(const cx<T>* a, const cx<T>* b, int nmodes, int N3, const T* w, double* partial,
+hipError_t launch_dot(const cx<T>* a, const cx<T>* b, int A1, int A2, int N3, const T* w, double* partial,
                       double* out, int slot, hipStream_t s);
 template <typename T>
 hipError_t launch_maxabs(const T* f, size_t n, int N3, const T* add, double* partial, double* out, int slot,
@@ -107,7 +97,7 @@ hipError_t launch_maxabs(const T* f, size_t n, int N3, const T* add, double* par
 template <typename T>
 hipError_t launch_axpby(cx<T>* y, T a, const cx<T>* x, T b, size_t n, hipStream_t s);
 template <typename T>
-hipError_t launch_ends(cx<T>* f, int nstacks, int N3, int bc_op, hipStream_t s);
+hipError_t launch_ends(cx<T>* f, int A1, int A2, int N3, int bc_op, hipStream_t s);
 template <typename T>
 hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, const T* kx, const T* ky, int axis,
                        const T* zc, int N3p, hipStream_t s);
@@ -116,7 +106,8 @@ bool tz_supported(int n);
 int xpass_levels(int N2, int N3);
 // twiddle table of an n-point LDS FFT (the layout its second pass reads)
 void fft_twiddles(int n, double* re, double* im);
-int ypass_max_lines(int N2, int precision);
 int max_rows_per_lane();
+// modes per block of the modal kernels for M rows per lane
+int modal_modes_per_block(int M, int precision);
 
 }  // namespace sf

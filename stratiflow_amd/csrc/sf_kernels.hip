@@ -25,7 +25,6 @@
 namespace sf {
 
 constexpr int BS = 256;
-static_assert(BS / 64 == BS_MODES, "modal kernels: one wavefront per mode");
 
 // ---------------------------------------------------------------------------
 // k_modal
@@ -118,18 +117,34 @@ __device__ __forceinline__ void build_rows(int kind, const ModalArgs<T>& A, cons
     }
 }
 
+// modes per block of the modal kernel: 8 consecutive j1 (one wavefront each)
+// when two such waves fit a SIMD's registers, else 4
 template <typename T, int M>
-__device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int mode) {
+constexpr int modal_mb() {
+    return (std::is_same<T, float>::value ? M <= 6 : M <= 3) ? 8 : 4;
+}
+
+// One block owns MB modes (ja, j10 .. j10+MB-1), one wavefront each.  Every
+// field access goes through an LDS tile of N3 rows x MB modes so that global
+// memory sees runs of MB consecutive j1 ([ja][z][j1] layout); wave w then
+// holds the column of mode j10+w with lane l owning rows l*M .. l*M+M-1.
+// Waves past A1 run on zeros and store nothing; they keep the block's barriers.
+template <typename T, int M, int MB>
+__device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, const int j10) {
     using V = cx<T>;
+    constexpr int TP = MB + 1;  // tile row pitch
+    __shared__ V tile[64 * M * TP];
     const int lane = threadIdx.x & 63;
-    if (mode >= A.nmodes) return;  // wave-uniform
-    const int j1 = mode % A.A1;
-    const int j2 = mode / A.A1;
-    const bool zero_mode = (mode == 0);
+    const int wave = threadIdx.x >> 6;
+    const int A1 = A.A1;
+    const bool active = j10 + wave < A1;
+    const int j1 = active ? j10 + wave : A1 - 1;
+    const int j2 = ja;
+    const bool zero_mode = (ja == 0 && j10 + wave == 0);
     const T kx = A.kx[j1], ky = A.ky[j2], kk = A.kx2[j1], kl = A.ky2[j2];
     const int N = A.N3;
-    const size_t base = size_t(mode) * size_t(N);
     const bool three = A.threeD != 0;
+    (void)active;
 
     int row[M];
     bool ok[M];
@@ -140,14 +155,33 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int mode
     }
     auto coefz = [&](int c, int t) -> T { return ok[t] ? A.zc[c * A.N3p + row[t]] : T(0); };
     auto load = [&](const V* f, V (&x)[M]) {
+        if (f == nullptr) {  // block-uniform
 #pragma unroll
-        for (int t = 0; t < M; ++t) x[t] = (f != nullptr && ok[t]) ? f[base + row[t]] : czero<V>();
+            for (int t = 0; t < M; ++t) x[t] = czero<V>();
+            return;
+        }
+        sync_lds();
+        for (int idx = threadIdx.x; idx < N * MB; idx += MB * 64) {
+            const int z = idx / MB, w = idx - (idx / MB) * MB;
+            const int jj = j10 + w;
+            tile[z * TP + w] = (jj < A1) ? f[sidx(ja, z, jj, N, A1)] : czero<V>();
+        }
+        sync_lds();
+#pragma unroll
+        for (int t = 0; t < M; ++t) x[t] = ok[t] ? tile[row[t] * TP + wave] : czero<V>();
     };
     auto store = [&](V* f, const V (&x)[M]) {
         if (f == nullptr) return;
+        sync_lds();
 #pragma unroll
         for (int t = 0; t < M; ++t)
-            if (ok[t]) f[base + row[t]] = x[t];
+            if (ok[t]) tile[row[t] * TP + wave] = x[t];
+        sync_lds();
+        for (int idx = threadIdx.x; idx < N * MB; idx += MB * 64) {
+            const int z = idx / MB, w = idx - (idx / MB) * MB;
+            const int jj = j10 + w;
+            if (jj < A1) f[sidx(ja, z, jj, N, A1)] = tile[z * TP + w];
+        }
     };
     // ZeroEnds, Field.h:475-486
     auto zero_ends = [&](V (&x)[M], bool dir) {
@@ -334,8 +368,8 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int mode
 }
 
 template <typename T, int M>
-__global__ __launch_bounds__(BS) void k_modal(ModalArgs<T> A) {
-    modal_body<T, M>(A, blockIdx.x * (BS / 64) + (threadIdx.x >> 6));
+__global__ __launch_bounds__((modal_mb<T, M>() * 64)) void k_modal(ModalArgs<T> A) {
+    modal_body<T, M, modal_mb<T, M>()>(A, blockIdx.y, blockIdx.x * modal_mb<T, M>());
 }
 
 // index r of the run owning work item `item`: start[r] <= item < start[r+1]
@@ -350,14 +384,31 @@ __device__ __forceinline__ int find_run(const int* __restrict__ start, int nruns
 }
 
 // Many independent runs in one launch: a grid-stride (persistent) loop over
-// the work items of all runs, BS/64 modes per item.  Every run has the same
-// rows-per-lane M; each reads its own arguments.
+// the work items of all runs, one item = one block's tile of modes.  Every
+// run has the same rows-per-lane M; each reads its own arguments.
 template <typename T, int M>
-__global__ __launch_bounds__(BS) void k_modal_batch(const ModalArgs<T>* __restrict__ args,
-                                                    const int* __restrict__ start, int nruns, int total) {
+__global__ __launch_bounds__((modal_mb<T, M>() * 64)) void k_modal_batch(const ModalArgs<T>* __restrict__ args,
+                                                                      const int* __restrict__ start, int nruns,
+                                                                      int total) {
+    constexpr int MB = modal_mb<T, M>();
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const int r = find_run(start, nruns, item);
-        modal_body<T, M>(args[r], (item - start[r]) * (BS / 64) + (threadIdx.x >> 6));
+        const ModalArgs<T>& a = args[r];
+        const int nt = (a.A1 + MB - 1) / MB;
+        const int local = item - start[r];
+        modal_body<T, M, MB>(a, local / nt, (local % nt) * MB);
+    }
+}
+
+int modal_modes_per_block(int M, int precision) {
+    switch (M) {
+#define SF_M(m) \
+    case m:     \
+        return precision == 4 ? modal_mb<float, m>() : modal_mb<double, m>();
+        SF_M(1) SF_M(2) SF_M(3) SF_M(4) SF_M(5) SF_M(6) SF_M(7) SF_M(8)
+#undef SF_M
+        default:
+            return 4;
     }
 }
 
@@ -366,9 +417,10 @@ hipError_t launch_modal_batch(const ModalArgs<T>* args, const int* start, int nr
                               hipStream_t s) {
     if (total <= 0) return hipSuccess;
     switch (M) {
-#define SF_M(m)                                                                                     \
-    case m:                                                                                         \
-        hipLaunchKernelGGL((k_modal_batch<T, m>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total); \
+#define SF_M(m)                                                                                               \
+    case m:                                                                                                   \
+        hipLaunchKernelGGL((k_modal_batch<T, m>), dim3(grid), dim3(modal_mb<T, m>() * 64), 0, s, args, start, \
+                           nruns, total);                                                                     \
         break;
         SF_M(1) SF_M(2) SF_M(3) SF_M(4) SF_M(5) SF_M(6) SF_M(7) SF_M(8)
 #undef SF_M
@@ -381,12 +433,13 @@ hipError_t launch_modal_batch(const ModalArgs<T>* args, const int* start, int nr
 template <typename T>
 hipError_t launch_modal(const ModalArgs<T>& a, hipStream_t s) {
     const int M = (a.N3 + 63) / 64;
-    dim3 grid((a.nmodes + BS / 64 - 1) / (BS / 64));
     switch (M) {
-#define SF_M(m)                                                     \
-    case m:                                                         \
-        hipLaunchKernelGGL((k_modal<T, m>), grid, dim3(BS), 0, s, a); \
-        break;
+#define SF_M(m)                                                                                       \
+    case m: {                                                                                         \
+        constexpr int MB = modal_mb<T, m>();                                                          \
+        hipLaunchKernelGGL((k_modal<T, m>), dim3((a.A1 + MB - 1) / MB, a.A2), dim3(MB * 64), 0, s, a); \
+        break;                                                                                        \
+    }
         SF_M(1) SF_M(2) SF_M(3) SF_M(4) SF_M(5) SF_M(6) SF_M(7) SF_M(8)
 #undef SF_M
         default:
@@ -454,7 +507,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
                 const int pair = item / A1;
                 const int j1 = item - pair * A1;
                 const V* const* in = (pair >= 2) ? A.base : A.in;
-                const size_t off = lay_at(A.lin, n2, j1, z);
+                const size_t off = sidx(n2, z, j1, N3, A1);
                 if ((pair & 1) == 0) {
                     X[i] = in[0][off];
                     Y[i] = in[2][off];
@@ -623,7 +676,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
             }
             const T kxv = A.kx[k];
             const T ms = -A.scale;
-            const size_t off = lay_at(A.lout, n2, k, z);
+            const size_t off = sidx(n2, z, k, N3, A1);
             // r1 -= ikx P1 + P2 ; r3 -= ikx P3 + ddz P4 ; rB -= P5 + ikx P6
             A.out[0][off] = cscale(ms, cadd(cik(kxv, Fa[0]), Fb[0]));
             A.out[2][off] = cscale(ms, cadd(cik(kxv, Fa[1]), Fb[1]));
@@ -726,136 +779,150 @@ void fft_twiddles(int n, double* re, double* im) {
 }
 
 // ---------------------------------------------------------------------------
-// y transforms (3D), one block per (z-tile of lay.zt levels, j1)
+// y transforms (3D), one block per (z, tile of YJ consecutive j1)
+//
+// The j1 tile makes both sides of the transpose between the modal layout
+// ([ja][z][j1]) and the x-spectral layout ([n2][z][j1]) read and write runs of
+// YJ consecutive values.  Lines (one per j1 of the tile and field) sit in LDS
+// at a pitch of 1 mod 32 complex values, so that the YJ lanes that fill or
+// drain the same position of YJ lines hit distinct banks.
 // ---------------------------------------------------------------------------
-template <int N, typename T>
-constexpr int ypass_lines() {
-    constexpr int lines = (53 * 1024) / (fft_ldp<N>() * int(sizeof(cx<T>)));
-    return lines < 8 ? 8 : (lines > 32 ? 32 : lines);
+template <int N>
+constexpr int y_ldp() {
+    int l = FftPlan<N>::two ? N + FftPlan<N>::R2 : N;
+    while (l % 32 != 1) ++l;
+    return l;
 }
 
-// inverse: out_f(n2, j1, z) = sum_{j2 active} in_f[j2a][j1][z] e^{+2 pi i j2 n2/N2}
-template <typename T, int N2, int NL>
+// inverse: out_f(n2, z, j1) = sum_{j2 active} in_f(ja, z, j1) e^{+2 pi i j2 n2/N2}
+template <typename T, int N2, int YJ, int FR>
 __global__ __launch_bounds__(BS) void k_yinv(YArgs<T> A) {
     using V = cx<T>;
-    constexpr int LDP = fft_ldp<N2>();
+    constexpr int LDP = y_ldp<N2>();
+    constexpr int NL = YJ * FR;
     __shared__ V buf[NL * LDP];
-    const int zt = A.lay.zt;
-    const int z0 = blockIdx.x * zt;
-    const int j1 = blockIdx.y;
+    const int z = blockIdx.x;
+    const int j10 = blockIdx.y * YJ;
     const int N3 = A.N3, A1 = A.A1, A2 = A.A2;
     const int h = N2 / 3;
-    const int fpr = NL / zt;  // fields per round
-    for (int f0 = 0; f0 < A.nf; f0 += fpr) {
+    auto lp = [&](int line) { return buf + line * LDP; };
+    for (int f0 = 0; f0 < A.nf; f0 += FR) {
         // zero band j2 = h .. N2-h of every line, then scatter the active modes
         const int nband = N2 - 2 * h + 1;
         for (int idx = threadIdx.x; idx < NL * nband; idx += BS) {
             const int line = idx / nband;
-            buf[line * LDP + (h + idx % nband)] = czero<V>();
+            buf[line * LDP + h + (idx - line * nband)] = czero<V>();
         }
-        for (int idx = threadIdx.x; idx < NL * A2; idx += BS) {
-            const int t = idx % zt;
-            const int rest = idx / zt;
+        for (int idx = threadIdx.x; idx < FR * A2 * YJ; idx += BS) {
+            const int jj = idx % YJ;
+            const int rest = idx / YJ;
             const int ja = rest % A2;
             const int fl = rest / A2;
             const int f = f0 + fl;
-            const int z = z0 + t;
+            const int j1 = j10 + jj;
             V v = czero<V>();
-            if (f < A.nf && z < N3) v = A.in[f][(size_t(ja) * A1 + j1) * N3 + z];
+            if (f < A.nf && j1 < A1) v = A.in[f][sidx(ja, z, j1, N3, A1)];
             const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
-            buf[(fl * zt + t) * LDP + (j2)] = v;
+            buf[(fl * YJ + jj) * LDP + j2] = v;
         }
         sync_lds();
-        lds_fft<T, N2, NL, BS, +1>(buf, A.tw);
-        for (int idx = threadIdx.x; idx < NL * N2; idx += BS) {
-            const int t = idx % zt;
-            const int rest = idx / zt;
+        lds_fft_lines<T, N2, NL, BS, +1>(NL, lp, A.tw);
+        for (int idx = threadIdx.x; idx < FR * N2 * YJ; idx += BS) {
+            const int jj = idx % YJ;
+            const int rest = idx / YJ;
             const int n2 = rest % N2;
             const int fl = rest / N2;
             const int f = f0 + fl;
-            const int z = z0 + t;
-            if (f < A.nf && z < N3) A.out[f][lay_at(A.lay, n2, j1, z)] = buf[(fl * zt + t) * LDP + (n2)];
+            const int j1 = j10 + jj;
+            if (f < A.nf && j1 < A1) A.out[f][sidx(n2, z, j1, N3, A1)] = buf[(fl * YJ + jj) * LDP + n2];
         }
         sync_lds();
     }
 }
 
-// forward: X_f(n2, j1, z) -> active j2.  paired: out[r] = scale*(F(in[2r]) + i ky F(in[2r+1]))
-template <typename T, int N2, int NL>
+// forward: X_f(n2, z, j1) -> active j2.  paired: out[r] = scale*(F(in[2r]) + i ky F(in[2r+1]))
+template <typename T, int N2, int YJ, int FR>
 __global__ __launch_bounds__(BS) void k_yfwd(YArgs<T> A) {
     using V = cx<T>;
-    constexpr int LDP = fft_ldp<N2>();
+    constexpr int LDP = y_ldp<N2>();
+    constexpr int NL = YJ * FR;
     __shared__ V buf[NL * LDP];
-    const int zt = A.lay.zt;
-    const int z0 = blockIdx.x * zt;
-    const int j1 = blockIdx.y;
+    const int z = blockIdx.x;
+    const int j10 = blockIdx.y * YJ;
     const int N3 = A.N3, A1 = A.A1, A2 = A.A2;
     const int h = N2 / 3;
-    const int fpr = NL / zt;  // input fields per round (even when paired)
     const int nin = A.paired ? 2 * A.nf : A.nf;
-    for (int f0 = 0; f0 < nin; f0 += fpr) {
-        for (int idx = threadIdx.x; idx < NL * N2; idx += BS) {
-            const int t = idx % zt;
-            const int rest = idx / zt;
+    auto lp = [&](int line) { return buf + line * LDP; };
+    for (int f0 = 0; f0 < nin; f0 += FR) {
+        for (int idx = threadIdx.x; idx < FR * N2 * YJ; idx += BS) {
+            const int jj = idx % YJ;
+            const int rest = idx / YJ;
             const int n2 = rest % N2;
             const int fl = rest / N2;
             const int f = f0 + fl;
-            const int z = z0 + t;
+            const int j1 = j10 + jj;
             V v = czero<V>();
-            if (f < nin && z < N3) v = A.in[f][lay_at(A.lay, n2, j1, z)];
-            buf[(fl * zt + t) * LDP + (n2)] = v;
+            if (f < nin && j1 < A1) v = A.in[f][sidx(n2, z, j1, N3, A1)];
+            buf[(fl * YJ + jj) * LDP + n2] = v;
         }
         sync_lds();
-        lds_fft<T, N2, NL, BS, -1>(buf, A.tw);
+        lds_fft_lines<T, N2, NL, BS, -1>(NL, lp, A.tw);
         if (A.paired) {
-            for (int idx = threadIdx.x; idx < (fpr / 2) * zt * A2; idx += BS) {
-                const int t = idx % zt;
-                const int rest = idx / zt;
+            for (int idx = threadIdx.x; idx < (FR / 2) * A2 * YJ; idx += BS) {
+                const int jj = idx % YJ;
+                const int rest = idx / YJ;
                 const int ja = rest % A2;
                 const int pr = rest / A2;
                 const int r = f0 / 2 + pr;
-                const int z = z0 + t;
-                if (r >= A.nf || z >= N3) continue;
+                const int j1 = j10 + jj;
+                if (r >= A.nf || j1 >= A1) continue;
                 const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
-                const V Fa = buf[((2 * pr) * zt + t) * LDP + (j2)];
-                const V Fb = buf[((2 * pr + 1) * zt + t) * LDP + (j2)];
-                A.out[r][(size_t(ja) * A1 + j1) * N3 + z] = cscale(A.scale, cadd(Fa, cik(A.ky[ja], Fb)));
+                const V Fa = buf[((2 * pr) * YJ + jj) * LDP + j2];
+                const V Fb = buf[((2 * pr + 1) * YJ + jj) * LDP + j2];
+                A.out[r][sidx(ja, z, j1, N3, A1)] = cscale(A.scale, cadd(Fa, cik(A.ky[ja], Fb)));
             }
         } else {
-            for (int idx = threadIdx.x; idx < fpr * zt * A2; idx += BS) {
-                const int t = idx % zt;
-                const int rest = idx / zt;
+            for (int idx = threadIdx.x; idx < FR * A2 * YJ; idx += BS) {
+                const int jj = idx % YJ;
+                const int rest = idx / YJ;
                 const int ja = rest % A2;
                 const int fl = rest / A2;
                 const int f = f0 + fl;
-                const int z = z0 + t;
-                if (f >= nin || z >= N3) continue;
+                const int j1 = j10 + jj;
+                if (f >= nin || j1 >= A1) continue;
                 const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
-                A.out[f][(size_t(ja) * A1 + j1) * N3 + z] = cscale(A.scale, buf[(fl * zt + t) * LDP + (j2)]);
+                A.out[f][sidx(ja, z, j1, N3, A1)] = cscale(A.scale, buf[(fl * YJ + jj) * LDP + j2]);
             }
         }
         sync_lds();
+    }
+}
+
+// tile shape: YJ = 16 j1 and FR = 2 fields per round when their lines fit in
+// 64 KB of LDS (two blocks per CU), else smaller tiles; the paired forward
+// pass needs FR even
+template <typename T, int N2, int YJ, int FR>
+static hipError_t launch_y_v(const YArgs<T>& a, bool inv, hipStream_t s) {
+    constexpr int lds = YJ * FR * y_ldp<N2>() * int(sizeof(cx<T>));
+    if constexpr (lds > 160 * 1024) {
+        return hipErrorInvalidValue;
+    } else {
+        dim3 grid(a.N3, (a.A1 + YJ - 1) / YJ);
+        if (inv) hipLaunchKernelGGL((k_yinv<T, N2, YJ, FR>), grid, dim3(BS), 0, s, a);
+        else hipLaunchKernelGGL((k_yfwd<T, N2, YJ, FR>), grid, dim3(BS), 0, s, a);
+        return hipGetLastError();
     }
 }
 
 template <typename T, int N2>
 static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
-    constexpr int maxl = ypass_lines<N2, T>();
-    const int zt = a.lay.zt;
-    // lines per block: a whole number of fields (pairs for the paired forward)
-    const int unit = (!inv && a.paired) ? 2 * zt : zt;
-    if (unit > maxl) return hipErrorInvalidValue;
-    dim3 grid((a.N3 + zt - 1) / zt, a.A1);
-    const int want = (maxl / unit) * unit;
-#define SF_NL(L)                                                                              \
-    if (want >= L && L % unit == 0) {                                                         \
-        if (inv) hipLaunchKernelGGL((k_yinv<T, N2, (L < maxl ? L : maxl)>), grid, dim3(BS), 0, s, a); \
-        else hipLaunchKernelGGL((k_yfwd<T, N2, (L < maxl ? L : maxl)>), grid, dim3(BS), 0, s, a);     \
-        return hipGetLastError();                                                             \
-    }
-    SF_NL(32) SF_NL(16) SF_NL(8) SF_NL(4) SF_NL(2)
-#undef SF_NL
-    return hipErrorInvalidValue;
+    constexpr int lb = y_ldp<N2>() * int(sizeof(cx<T>));
+    const bool paired = !inv && a.paired;
+    if (32 * lb <= 64 * 1024) return launch_y_v<T, N2, 16, 2>(a, inv, s);
+    if (!paired && 16 * lb <= 64 * 1024) return launch_y_v<T, N2, 16, 1>(a, inv, s);
+    if (16 * lb <= 64 * 1024) return launch_y_v<T, N2, 8, 2>(a, inv, s);
+    if (!paired) return launch_y_v<T, N2, 8, 1>(a, inv, s);
+    return launch_y_v<T, N2, 4, 2>(a, inv, s);
 }
 
 template <typename T>
@@ -883,21 +950,9 @@ hipError_t launch_yfwd(const YArgs<T>& a, int N2, hipStream_t s) {
     }
 }
 
-int ypass_max_lines(int N2, int precision) {
-    switch (N2) {
-#define SF_CASE(n) \
-    case n:        \
-        return precision == 4 ? ypass_lines<n, float>() : ypass_lines<n, double>();
-        SF_SIZES(SF_CASE)
-#undef SF_CASE
-        default:
-            return 0;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // single-field x transforms (ToNodal / ToModal for API fields), one block per
-// (z-tile of lay.zt levels, n2)
+// (range of nzt z levels, n2)
 // ---------------------------------------------------------------------------
 template <typename T, int N1, int NL>
 __global__ __launch_bounds__(BS) void k_x_c2r(XTArgs<T> A) {
@@ -911,11 +966,11 @@ __global__ __launch_bounds__(BS) void k_x_c2r(XTArgs<T> A) {
         for (int idx = threadIdx.x; idx < NL * nband; idx += BS)
             buf[(idx / nband) * LDP + (A1 + idx % nband)] = czero<V>();
         for (int idx = threadIdx.x; idx < NL * A1; idx += BS) {
-            const int t = idx % NL;
-            const int j1 = idx / NL;
+            const int j1 = idx % A1;
+            const int t = idx / A1;
             const int z = z0 + t;
             V X = czero<V>();
-            if (z < N3) X = A.spec_in[lay_at(A.lay, n2, j1, z)];
+            if (z < N3) X = A.spec_in[sidx(n2, z, j1, N3, A1)];
             V* L = buf + t * LDP;
             if (j1 == 0) {
                 L[(0)] = mkc<T>(X.x, T(0));
@@ -955,10 +1010,10 @@ __global__ __launch_bounds__(BS) void k_x_r2c(XTArgs<T> A) {
         sync_lds();
         lds_fft<T, N1, NL, BS, -1>(buf, A.tw);
         for (int idx = threadIdx.x; idx < NL * A1; idx += BS) {
-            const int t = idx % NL;
-            const int k = idx / NL;
+            const int k = idx % A1;
+            const int t = idx / A1;
             const int z = z0 + t;
-            if (z < N3) A.spec_out[lay_at(A.lay, n2, k, z)] = cscale(A.scale, buf[t * LDP + (k)]);
+            if (z < N3) A.spec_out[sidx(n2, z, k, N3, A1)] = cscale(A.scale, buf[t * LDP + (k)]);
         }
         sync_lds();
     }
@@ -1011,16 +1066,18 @@ constexpr int DOT_BLOCKS = 1024;
 
 template <typename T>
 __global__ __launch_bounds__(BS) void k_dot_partial(const cx<T>* __restrict__ a, const cx<T>* __restrict__ b,
-                                                    size_t n, int N3, const T* __restrict__ w,
+                                                    size_t n, int N3, int A1, const T* __restrict__ w,
                                                     double* __restrict__ partial) {
     __shared__ double red[BS];
     double acc = 0;
     for (size_t e = size_t(blockIdx.x) * BS + threadIdx.x; e < n; e += size_t(gridDim.x) * BS) {
-        const int z = int(e % size_t(N3));
-        const size_t mode = e / size_t(N3);
+        const size_t rowz = e / size_t(A1);  // [ja][z][j1]
+        const int j1 = int(e - rowz * size_t(A1));
+        const int z = int(rowz % size_t(N3));
+        const bool m00 = (j1 == 0) && (rowz < size_t(N3));  // ja == 0 and j1 == 0
         const cx<T> x = a[e], y = b[e];
         const double re = double(x.x) * double(y.x) + double(x.y) * double(y.y);
-        acc += (mode == 0 ? 1.0 : 2.0) * double(w[z]) * re;
+        acc += (m00 ? 1.0 : 2.0) * double(w[z]) * re;
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -1045,13 +1102,13 @@ __global__ __launch_bounds__(BS) void k_dot_final(const double* __restrict__ par
 }
 
 template <typename T>
-hipError_t launch_dot(const cx<T>* a, const cx<T>* b, int nmodes, int N3, const T* w, double* partial, double* out,
-                      int slot, hipStream_t s) {
-    const size_t n = size_t(nmodes) * size_t(N3);
+hipError_t launch_dot(const cx<T>* a, const cx<T>* b, int A1, int A2, int N3, const T* w, double* partial,
+                      double* out, int slot, hipStream_t s) {
+    const size_t n = size_t(A1) * size_t(A2) * size_t(N3);
     int blocks = int((n + BS - 1) / BS);
     if (blocks > DOT_BLOCKS) blocks = DOT_BLOCKS;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL((k_dot_partial<T>), dim3(blocks), dim3(BS), 0, s, a, b, n, N3, w, partial);
+    hipLaunchKernelGGL((k_dot_partial<T>), dim3(blocks), dim3(BS), 0, s, a, b, n, N3, A1, w, partial);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(BS), 0, s, partial, blocks, out, slot);
     return hipGetLastError();
 }
@@ -1124,21 +1181,22 @@ hipError_t launch_axpby(cx<T>* y, T a, const cx<T>* x, T b, size_t n, hipStream_
 
 // bc_op: 0 ZeroEnds Neumann, 1 ZeroEnds Dirichlet, 2 NeumannEnds (Field.h:475-500)
 template <typename T>
-__global__ __launch_bounds__(BS) void k_ends(cx<T>* f, int nstacks, int N3, int op) {
-    const int s = blockIdx.x * BS + threadIdx.x;
-    if (s >= nstacks) return;
-    cx<T>* c = f + size_t(s) * N3;
+__global__ __launch_bounds__(BS) void k_ends(cx<T>* f, int A1, int A2, int N3, int op) {
+    const int s = blockIdx.x * BS + threadIdx.x;  // stack (ja, j1), j1 fastest
+    if (s >= A1 * A2) return;
+    const int ja = s / A1, j1 = s - (s / A1) * A1;
+    auto at = [&](int z) -> cx<T>& { return f[sidx(ja, z, j1, N3, A1)]; };
     if (op == 2) {
-        c[0] = c[1];
-        c[N3 - 1] = c[N3 - 2];
+        at(0) = at(1);
+        at(N3 - 1) = at(N3 - 2);
     } else {
-        c[0] = c[1] = c[N3 - 1] = c[N3 - 2] = czero<cx<T>>();
-        if (op == 1) c[2] = czero<cx<T>>();
+        at(0) = at(1) = at(N3 - 1) = at(N3 - 2) = czero<cx<T>>();
+        if (op == 1) at(2) = czero<cx<T>>();
     }
 }
 template <typename T>
-hipError_t launch_ends(cx<T>* f, int nstacks, int N3, int op, hipStream_t s) {
-    hipLaunchKernelGGL((k_ends<T>), dim3((nstacks + BS - 1) / BS), dim3(BS), 0, s, f, nstacks, N3, op);
+hipError_t launch_ends(cx<T>* f, int A1, int A2, int N3, int op, hipStream_t s) {
+    hipLaunchKernelGGL((k_ends<T>), dim3((A1 * A2 + BS - 1) / BS), dim3(BS), 0, s, f, A1, A2, N3, op);
     return hipGetLastError();
 }
 
@@ -1149,18 +1207,18 @@ __global__ __launch_bounds__(BS) void k_dmul(cx<T>* out, const cx<T>* in, int A1
                                              const T* ky, int axis, const T* zc, int N3p) {
     const size_t n = size_t(A1) * A2 * N3;
     for (size_t e = size_t(blockIdx.x) * BS + threadIdx.x; e < n; e += size_t(gridDim.x) * BS) {
-        const int z = int(e % size_t(N3));
-        const size_t mode = e / size_t(N3);
-        const int j1 = int(mode % size_t(A1));
-        const int j2 = int(mode / size_t(A1));
+        const size_t rowz = e / size_t(A1);  // [ja][z][j1]
+        const int j1 = int(e - rowz * size_t(A1));
+        const int z = int(rowz % size_t(N3));
+        const int j2 = int(rowz / size_t(N3));
         cx<T> r;
         if (axis == 0) r = cik(kx[j1], in[e]);
         else if (axis == 1) r = cik(ky[j2], in[e]);
         else if (axis == 2) {  // Neumann in -> Dirichlet out
-            const cx<T> fm = z > 0 ? in[e - 1] : czero<cx<T>>();
+            const cx<T> fm = z > 0 ? in[e - size_t(A1)] : czero<cx<T>>();
             r = cscale(zc[ZC_DZND * N3p + z], csub(in[e], fm));
         } else {  // Dirichlet in -> Neumann out
-            const cx<T> fp = z + 1 < N3 ? in[e + 1] : czero<cx<T>>();
+            const cx<T> fp = z + 1 < N3 ? in[e + size_t(A1)] : czero<cx<T>>();
             r = cscale(zc[ZC_DZDN * N3p + z], csub(fp, in[e]));
         }
         out[e] = r;
@@ -1187,11 +1245,11 @@ hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, cons
     template hipError_t launch_yfwd<T>(const YArgs<T>&, int, hipStream_t);                                           \
     template hipError_t launch_x_c2r<T>(const XTArgs<T>&, int, int, hipStream_t);                                   \
     template hipError_t launch_x_r2c<T>(const XTArgs<T>&, int, int, hipStream_t);                                   \
-    template hipError_t launch_dot<T>(const cx<T>*, const cx<T>*, int, int, const T*, double*, double*, int,         \
+    template hipError_t launch_dot<T>(const cx<T>*, const cx<T>*, int, int, int, const T*, double*, double*, int,    \
                                       hipStream_t);                                                                  \
     template hipError_t launch_axpby<T>(cx<T>*, T, const cx<T>*, T, size_t, hipStream_t);                          \
     template hipError_t launch_maxabs<T>(const T*, size_t, int, const T*, double*, double*, int, hipStream_t);       \
-    template hipError_t launch_ends<T>(cx<T>*, int, int, int, hipStream_t);                                          \
+    template hipError_t launch_ends<T>(cx<T>*, int, int, int, int, hipStream_t);                                          \
     template hipError_t launch_dmul<T>(cx<T>*, const cx<T>*, int, int, int, const T*, const T*, int, const T*, int, \
                                        hipStream_t);
 
