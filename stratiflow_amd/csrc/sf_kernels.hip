@@ -117,11 +117,11 @@ __device__ __forceinline__ void build_rows(int kind, const ModalArgs<T>& A, cons
     }
 }
 
-// modes per block of the modal kernel: 8 consecutive j1 (one wavefront each)
-// when two such waves fit a SIMD's registers, else 4
+// modes per block of the modal kernel: 4 consecutive j1, one wavefront each
+// (two such blocks share a CU at the register use of M = 6)
 template <typename T, int M>
 constexpr int modal_mb() {
-    return (std::is_same<T, float>::value ? M <= 6 : M <= 3) ? 8 : 4;
+    return 4;
 }
 
 // One block owns MB modes (ja, j10 .. j10+MB-1), one wavefront each.  Every
@@ -154,17 +154,33 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         ok[t] = row[t] < N;
     }
     auto coefz = [&](int c, int t) -> T { return ok[t] ? A.zc[c * A.N3p + row[t]] : T(0); };
-    auto load = [&](const V* f, V (&x)[M]) {
+    // tile element owned by this thread in the cooperative (coalesced) phase:
+    // rows z = tid/MB + t*64 for t < M (N3 <= 64*M), mode w = tid % MB
+    const int cw = int(threadIdx.x) % MB;
+    const int cz = int(threadIdx.x) / MB;
+    const int cj1 = j10 + cw;
+    // issue: global loads of this thread's tile elements into g (left in
+    // flight); finish: through the LDS tile into the wave's column x.  Several
+    // issues before their finishes overlap the memory latency of those fields.
+    auto issue = [&](const V* f, V (&g)[M]) {
+        if (f == nullptr) return;  // block-uniform
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            const int z = cz + t * 64;
+            g[t] = (z < N && cj1 < A1) ? f[sidx(ja, z, cj1, N, A1)] : czero<V>();
+        }
+    };
+    auto finish = [&](const V* f, const V (&g)[M], V (&x)[M]) {
         if (f == nullptr) {  // block-uniform
 #pragma unroll
             for (int t = 0; t < M; ++t) x[t] = czero<V>();
             return;
         }
-        sync_lds();
-        for (int idx = threadIdx.x; idx < N * MB; idx += MB * 64) {
-            const int z = idx / MB, w = idx - (idx / MB) * MB;
-            const int jj = j10 + w;
-            tile[z * TP + w] = (jj < A1) ? f[sidx(ja, z, jj, N, A1)] : czero<V>();
+        sync_lds();  // the previous tile has been read by every wave
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            const int z = cz + t * 64;
+            if (z < N) tile[z * TP + cw] = g[t];
         }
         sync_lds();
 #pragma unroll
@@ -177,10 +193,10 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         for (int t = 0; t < M; ++t)
             if (ok[t]) tile[row[t] * TP + wave] = x[t];
         sync_lds();
-        for (int idx = threadIdx.x; idx < N * MB; idx += MB * 64) {
-            const int z = idx / MB, w = idx - (idx / MB) * MB;
-            const int jj = j10 + w;
-            if (jj < A1) f[sidx(ja, z, jj, N, A1)] = tile[z * TP + w];
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            const int z = cz + t * 64;
+            if (z < N && cj1 < A1) f[sidx(ja, z, cj1, N, A1)] = tile[z * TP + cw];
         }
     };
     // ZeroEnds, Field.h:475-486
@@ -192,22 +208,32 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         }
     };
 
+    V g0[M], g1[M], g2[M], g3[M];
     V u1[M], u2[M], u3[M], bb[M], pp[M];
-    load(A.u1, u1);
-    if (three) load(A.u2, u2);
-    else
-#pragma unroll
-        for (int t = 0; t < M; ++t) u2[t] = czero<V>();
-    load(A.u3, u3);
-    load(A.p, pp);
+    const V* pu2 = three ? A.u2 : nullptr;
+    issue(A.u1, g0);
+    issue(pu2, g1);
+    issue(A.u3, g2);
+    issue(A.p, g3);
+    finish(A.u1, g0, u1);
+    finish(pu2, g1, u2);
+    finish(A.u3, g2, u3);
+    finish(A.p, g3, pp);
 
     if (A.mode == 0) {
-        load(A.b, bb);
         V rn1[M], rn2[M], rn3[M], rnB[M];
-        load(A.n1, rn1);
-        if (three) load(A.n2, rn2);
-        load(A.n3, rn3);
-        load(A.nB, rnB);
+        const V* pn2 = three ? A.n2 : nullptr;
+        issue(A.b, g0);
+        issue(A.n1, g1);
+        issue(pn2, g2);
+        issue(A.n3, g3);
+        finish(A.b, g0, bb);
+        finish(A.n1, g1, rn1);
+        finish(pn2, g2, rn2);
+        finish(A.n3, g3, rn3);
+        issue(A.nB, g0);
+        finish(A.nB, g0, rnB);
+        issue(A.ro1, g0);  // first explicit term of the previous stage, in flight from here
         // BuildRHS linear terms, IMEXRK.cpp:140-147:
         //   r3 += Ri * ReinterpolateFull(b - <b>),  rB -= S * ReinterpolateDirichlet(u3)
         {
@@ -233,11 +259,12 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         store(A.rn3, rn3);
         store(A.rnB, rnB);
 
-        // ExplicitRK + FinishRHS + CrankNicolson explicit half, IMEXRK.cpp:75-134
-        auto assemble = [&](V (&R)[M], const V (&u)[M], const V* ro, const V (&rn)[M], const V (&gp)[M], bool dir,
-                            T c) {
+        // ExplicitRK + FinishRHS + CrankNicolson explicit half, IMEXRK.cpp:75-134;
+        // ro (the previous stage's explicit term) was issued into g earlier
+        auto assemble = [&](V (&R)[M], const V (&u)[M], const V* ro, const V (&g)[M], const V (&rn)[M],
+                            const V (&gp)[M], bool dir, T c) {
             V ro_[M];
-            load(ro, ro_);
+            finish(ro, g, ro_);
             V um[M], up[M];
             nb_prev<T, M>(u, um, lane);
             nb_next<T, M>(u, up, lane);
@@ -262,14 +289,17 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         tri_factor<T, M>(F, ta, tb, tc, lane);
 #pragma unroll
         for (int t = 0; t < M; ++t) gp[t] = cik(kx, pp[t]);
-        assemble(R, u1, A.ro1, rn1, gp, false, A.cv);
+        assemble(R, u1, A.ro1, g0, rn1, gp, false, A.cv);
+        const V* pro2 = three ? A.ro2 : nullptr;
+        issue(pro2, g1);
+        issue(A.ro3, g2);
         tri_apply<T, M>(F, R, lane);
 #pragma unroll
         for (int t = 0; t < M; ++t) u1[t] = R[t];
         if (three) {
 #pragma unroll
             for (int t = 0; t < M; ++t) gp[t] = cik(ky, pp[t]);
-            assemble(R, u2, A.ro2, rn2, gp, false, A.cv);
+            assemble(R, u2, pro2, g1, rn2, gp, false, A.cv);
             tri_apply<T, M>(F, R, lane);
 #pragma unroll
             for (int t = 0; t < M; ++t) u2[t] = R[t];
@@ -283,7 +313,8 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         }
         build_rows<T, M>(1, A, row, kk, kl, zero_mode, A.cv, ta, tb, tc);
         tri_factor<T, M>(F, ta, tb, tc, lane);
-        assemble(R, u3, A.ro3, rn3, gp, true, A.cv);
+        assemble(R, u3, A.ro3, g2, rn3, gp, true, A.cv);
+        issue(A.roB, g3);
         tri_apply<T, M>(F, R, lane);
 #pragma unroll
         for (int t = 0; t < M; ++t) u3[t] = R[t];
@@ -292,7 +323,7 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         for (int t = 0; t < M; ++t) gp[t] = czero<V>();
         build_rows<T, M>(2, A, row, kk, kl, zero_mode, A.cb, ta, tb, tc);
         tri_factor<T, M>(F, ta, tb, tc, lane);
-        assemble(R, bb, A.roB, rnB, gp, false, A.cb);
+        assemble(R, bb, A.roB, g3, rnB, gp, false, A.cb);
         tri_apply<T, M>(F, R, lane);
 #pragma unroll
         for (int t = 0; t < M; ++t) bb[t] = R[t];
@@ -367,9 +398,21 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
     store(A.p, pp);
 }
 
+// Workgroups go to the XCDs round-robin (block b to XCD b % 8), each XCD with
+// its own L2.  Blocks whose tiles share cache lines (neighbouring j1 tiles)
+// are renumbered so that they land on the same XCD at about the same time:
+// XCD x takes the x-th contiguous eighth of the tile sequence.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    constexpr int NXCD = 8;
+    if (nb % NXCD) return b;
+    return (b % NXCD) * (nb / NXCD) + b / NXCD;
+}
+
 template <typename T, int M>
 __global__ __launch_bounds__((modal_mb<T, M>() * 64)) void k_modal(ModalArgs<T> A) {
-    modal_body<T, M, modal_mb<T, M>()>(A, blockIdx.y, blockIdx.x * modal_mb<T, M>());
+    const int nt = gridDim.x;  // j1 tiles per ja
+    const int t = xcd_tile(blockIdx.x + nt * blockIdx.y, nt * gridDim.y);
+    modal_body<T, M, modal_mb<T, M>()>(A, t / nt, (t % nt) * modal_mb<T, M>());
 }
 
 // index r of the run owning work item `item`: start[r] <= item < start[r+1]
@@ -801,8 +844,10 @@ __global__ __launch_bounds__(BS) void k_yinv(YArgs<T> A) {
     constexpr int LDP = y_ldp<N2>();
     constexpr int NL = YJ * FR;
     __shared__ V buf[NL * LDP];
-    const int z = blockIdx.x;
-    const int j10 = blockIdx.y * YJ;
+    // dispatch index -> (z, j1 tile) with neighbouring j1 tiles on one XCD
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int z = tile / gridDim.y;
+    const int j10 = (tile % gridDim.y) * YJ;
     const int N3 = A.N3, A1 = A.A1, A2 = A.A2;
     const int h = N2 / 3;
     auto lp = [&](int line) { return buf + line * LDP; };
@@ -847,8 +892,10 @@ __global__ __launch_bounds__(BS) void k_yfwd(YArgs<T> A) {
     constexpr int LDP = y_ldp<N2>();
     constexpr int NL = YJ * FR;
     __shared__ V buf[NL * LDP];
-    const int z = blockIdx.x;
-    const int j10 = blockIdx.y * YJ;
+    // dispatch index -> (z, j1 tile) with neighbouring j1 tiles on one XCD
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int z = tile / gridDim.y;
+    const int j10 = (tile % gridDim.y) * YJ;
     const int N3 = A.N3, A1 = A.A1, A2 = A.A2;
     const int h = N2 / 3;
     const int nin = A.paired ? 2 * A.nf : A.nf;
@@ -898,9 +945,6 @@ __global__ __launch_bounds__(BS) void k_yfwd(YArgs<T> A) {
     }
 }
 
-// tile shape: YJ = 16 j1 and FR = 2 fields per round when their lines fit in
-// 64 KB of LDS (two blocks per CU), else smaller tiles; the paired forward
-// pass needs FR even
 template <typename T, int N2, int YJ, int FR>
 static hipError_t launch_y_v(const YArgs<T>& a, bool inv, hipStream_t s) {
     constexpr int lds = YJ * FR * y_ldp<N2>() * int(sizeof(cx<T>));
@@ -916,13 +960,11 @@ static hipError_t launch_y_v(const YArgs<T>& a, bool inv, hipStream_t s) {
 
 template <typename T, int N2>
 static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
-    constexpr int lb = y_ldp<N2>() * int(sizeof(cx<T>));
-    const bool paired = !inv && a.paired;
-    if (32 * lb <= 64 * 1024) return launch_y_v<T, N2, 16, 2>(a, inv, s);
-    if (!paired && 16 * lb <= 64 * 1024) return launch_y_v<T, N2, 16, 1>(a, inv, s);
-    if (16 * lb <= 64 * 1024) return launch_y_v<T, N2, 8, 2>(a, inv, s);
-    if (!paired) return launch_y_v<T, N2, 8, 1>(a, inv, s);
-    return launch_y_v<T, N2, 4, 2>(a, inv, s);
+    // 16 lines per block keeps registers and LDS low enough for several
+    // blocks per CU: 16 j1 x 1 field, or 8 j1 x 2 fields for the paired
+    // forward pass
+    if (inv || !a.paired) return launch_y_v<T, N2, 16, 1>(a, inv, s);
+    return launch_y_v<T, N2, 8, 2>(a, inv, s);
 }
 
 template <typename T>
