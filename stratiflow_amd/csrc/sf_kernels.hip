@@ -830,19 +830,24 @@ void fft_twiddles(int n, double* re, double* im) {
 // at a pitch of 1 mod 32 complex values, so that the YJ lanes that fill or
 // drain the same position of YJ lines hit distinct banks.
 // ---------------------------------------------------------------------------
-template <int N>
+// line pitch: a wave fills or drains 64/YJ positions of YJ lines at once;
+// a pitch of 64/YJ mod 16 complex spreads those lanes evenly over the banks
+template <int N, int YJ>
 constexpr int y_ldp() {
     int l = FftPlan<N>::two ? N + FftPlan<N>::R2 : N;
-    while (l % 32 != 1) ++l;
+    while (l % 16 != (64 / YJ) % 16) ++l;
     return l;
 }
 
 // inverse: out_f(n2, z, j1) = sum_{j2 active} in_f(ja, z, j1) e^{+2 pi i j2 n2/N2}
+// The loads of the next field are issued before the FFT of the current one.
 template <typename T, int N2, int YJ, int FR>
 __global__ __launch_bounds__(BS) void k_yinv(YArgs<T> A) {
     using V = cx<T>;
-    constexpr int LDP = y_ldp<N2>();
+    constexpr int LDP = y_ldp<N2, YJ>();
     constexpr int NL = YJ * FR;
+    constexpr int AMAX = 2 * (N2 / 3);  // bound on A2
+    constexpr int PG = (FR * AMAX * YJ + BS - 1) / BS;
     __shared__ V buf[NL * LDP];
     // dispatch index -> (z, j1 tile) with neighbouring j1 tiles on one XCD
     const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -851,26 +856,42 @@ __global__ __launch_bounds__(BS) void k_yinv(YArgs<T> A) {
     const int N3 = A.N3, A1 = A.A1, A2 = A.A2;
     const int h = N2 / 3;
     auto lp = [&](int line) { return buf + line * LDP; };
-    for (int f0 = 0; f0 < A.nf; f0 += FR) {
-        // zero band j2 = h .. N2-h of every line, then scatter the active modes
-        const int nband = N2 - 2 * h + 1;
-        for (int idx = threadIdx.x; idx < NL * nband; idx += BS) {
-            const int line = idx / nband;
-            buf[line * LDP + h + (idx - line * nband)] = czero<V>();
-        }
-        for (int idx = threadIdx.x; idx < FR * A2 * YJ; idx += BS) {
+    V g[PG];
+    auto issue = [&](int f0) {
+#pragma unroll
+        for (int i = 0; i < PG; ++i) {
+            const int idx = int(threadIdx.x) + i * BS;
             const int jj = idx % YJ;
             const int rest = idx / YJ;
             const int ja = rest % A2;
             const int fl = rest / A2;
             const int f = f0 + fl;
             const int j1 = j10 + jj;
-            V v = czero<V>();
-            if (f < A.nf && j1 < A1) v = A.in[f][sidx(ja, z, j1, N3, A1)];
-            const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
-            buf[(fl * YJ + jj) * LDP + j2] = v;
+            g[i] = (idx < FR * A2 * YJ && f < A.nf && j1 < A1) ? A.in[f][sidx(ja, z, j1, N3, A1)] : czero<V>();
+        }
+    };
+    issue(0);
+    for (int f0 = 0; f0 < A.nf; f0 += FR) {
+        // zero band j2 = h .. N2-h of every line, then place the active modes
+        const int nband = N2 - 2 * h + 1;
+        for (int idx = threadIdx.x; idx < NL * nband; idx += BS) {
+            const int line = idx / nband;
+            buf[line * LDP + h + (idx - line * nband)] = czero<V>();
+        }
+#pragma unroll
+        for (int i = 0; i < PG; ++i) {
+            const int idx = int(threadIdx.x) + i * BS;
+            if (idx < FR * A2 * YJ) {
+                const int jj = idx % YJ;
+                const int rest = idx / YJ;
+                const int ja = rest % A2;
+                const int fl = rest / A2;
+                const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
+                buf[(fl * YJ + jj) * LDP + j2] = g[i];
+            }
         }
         sync_lds();
+        if (f0 + FR < A.nf) issue(f0 + FR);
         lds_fft_lines<T, N2, NL, BS, +1>(NL, lp, A.tw);
         for (int idx = threadIdx.x; idx < FR * N2 * YJ; idx += BS) {
             const int jj = idx % YJ;
@@ -889,8 +910,9 @@ __global__ __launch_bounds__(BS) void k_yinv(YArgs<T> A) {
 template <typename T, int N2, int YJ, int FR>
 __global__ __launch_bounds__(BS) void k_yfwd(YArgs<T> A) {
     using V = cx<T>;
-    constexpr int LDP = y_ldp<N2>();
+    constexpr int LDP = y_ldp<N2, YJ>();
     constexpr int NL = YJ * FR;
+    constexpr int PG = (FR * N2 * YJ + BS - 1) / BS;
     __shared__ V buf[NL * LDP];
     // dispatch index -> (z, j1 tile) with neighbouring j1 tiles on one XCD
     const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -900,19 +922,35 @@ __global__ __launch_bounds__(BS) void k_yfwd(YArgs<T> A) {
     const int h = N2 / 3;
     const int nin = A.paired ? 2 * A.nf : A.nf;
     auto lp = [&](int line) { return buf + line * LDP; };
-    for (int f0 = 0; f0 < nin; f0 += FR) {
-        for (int idx = threadIdx.x; idx < FR * N2 * YJ; idx += BS) {
+    V g[PG];
+    auto issue = [&](int f0) {
+#pragma unroll
+        for (int i = 0; i < PG; ++i) {
+            const int idx = int(threadIdx.x) + i * BS;
             const int jj = idx % YJ;
             const int rest = idx / YJ;
             const int n2 = rest % N2;
             const int fl = rest / N2;
             const int f = f0 + fl;
             const int j1 = j10 + jj;
-            V v = czero<V>();
-            if (f < nin && j1 < A1) v = A.in[f][sidx(n2, z, j1, N3, A1)];
-            buf[(fl * YJ + jj) * LDP + n2] = v;
+            g[i] = (idx < FR * N2 * YJ && f < nin && j1 < A1) ? A.in[f][sidx(n2, z, j1, N3, A1)] : czero<V>();
+        }
+    };
+    issue(0);
+    for (int f0 = 0; f0 < nin; f0 += FR) {
+#pragma unroll
+        for (int i = 0; i < PG; ++i) {
+            const int idx = int(threadIdx.x) + i * BS;
+            if (idx < FR * N2 * YJ) {
+                const int jj = idx % YJ;
+                const int rest = idx / YJ;
+                const int n2 = rest % N2;
+                const int fl = rest / N2;
+                buf[(fl * YJ + jj) * LDP + n2] = g[i];
+            }
         }
         sync_lds();
+        if (f0 + FR < nin) issue(f0 + FR);
         lds_fft_lines<T, N2, NL, BS, -1>(NL, lp, A.tw);
         if (A.paired) {
             for (int idx = threadIdx.x; idx < (FR / 2) * A2 * YJ; idx += BS) {
@@ -947,7 +985,7 @@ __global__ __launch_bounds__(BS) void k_yfwd(YArgs<T> A) {
 
 template <typename T, int N2, int YJ, int FR>
 static hipError_t launch_y_v(const YArgs<T>& a, bool inv, hipStream_t s) {
-    constexpr int lds = YJ * FR * y_ldp<N2>() * int(sizeof(cx<T>));
+    constexpr int lds = YJ * FR * y_ldp<N2, YJ>() * int(sizeof(cx<T>));
     if constexpr (lds > 160 * 1024) {
         return hipErrorInvalidValue;
     } else {
