@@ -518,10 +518,26 @@ bool tz_supported(int n) {
     }
 }
 
+// line pitch of the x-pass: the smallest that holds the FFT's padded pass-1
+// table, rounded to 8 mod 32 complex values (keeps four 3D blocks per CU)
+template <int N>
+constexpr int x_ldp() {
+    int l = FftPlan<N>::two ? N + FftPlan<N>::R2 : N + 1;
+    while (l % 32 != 8) ++l;
+    return l;
+}
+
+// waves per SIMD the x-pass is compiled for: four in FP32 3D without the
+// linearised products (register budget 128), else what the registers allow
+template <typename T, bool THREED, bool LIN>
+constexpr int xpass_waves() {
+    return (std::is_same<T, float>::value && THREED && !LIN) ? 4 : 1;
+}
+
 template <typename T, int N1, bool THREED, bool LIN>
 __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, const int n2) {
     using V = cx<T>;
-    constexpr int LDP = fft_ldp<N1>();
+    constexpr int LDP = x_ldp<N1>();
     constexpr int NI = LIN ? 4 : 2;        // input pair lines per level
     constexpr int NP = THREED ? 6 : 3;     // product pair lines per level
     constexpr int NB = NP + NI;            // lines per batched FFT
@@ -738,7 +754,8 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
 }
 
 template <typename T, int N1, bool THREED, bool LIN>
-__global__ __launch_bounds__(BS) void k_xpass(XArgs<T> A) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(xpass_waves<T, THREED, LIN>()))) void k_xpass(
+    XArgs<T> A) {
     xpass_body<T, N1, THREED, LIN>(A, blockIdx.x, blockIdx.y);
 }
 
@@ -771,7 +788,7 @@ hipError_t launch_xpass_batch(const XArgs<T>* args, const int* start, int nruns,
 
 template <typename T, int N1, bool THREED, bool LIN>
 static hipError_t launch_xpass_v(const XArgs<T>& a, int N2, hipStream_t s) {
-    constexpr int lds = ((THREED ? 6 : 3) + 3 * (LIN ? 4 : 2)) * fft_ldp<N1>() * int(sizeof(cx<T>));
+    constexpr int lds = ((THREED ? 6 : 3) + 3 * (LIN ? 4 : 2)) * x_ldp<N1>() * int(sizeof(cx<T>));
     if constexpr (lds > 160 * 1024) {
         return hipErrorInvalidValue;  // not instantiated: does not fit the LDS of one CU
     } else {
