@@ -121,7 +121,7 @@ __device__ __forceinline__ void build_rows(int kind, const ModalArgs<T>& A, cons
 // (two such blocks share a CU at the register use of M = 6)
 template <typename T, int M>
 constexpr int modal_mb() {
-    return 4;
+    return (std::is_same<T, float>::value ? M <= 6 : M <= 3) ? 8 : 4;
 }
 
 // One block owns MB modes (ja, j10 .. j10+MB-1), one wavefront each.  Every
