@@ -116,7 +116,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=40.0, help="budget for the CPU baseline sample")
-    ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_c2.json"))
+    ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_384x192x384_fp32.json"))
     ap.add_argument("--calibration-copies", type=int, default=0,
                     help="device-to-device copies of one modal field (known bytes) after the timed region, "
                          "to scale FETCH_SIZE/WRITE_SIZE counters")
