@@ -202,7 +202,7 @@ struct Ctx {
         A1 = N1 / 3;
         A2 = three ? 2 * (N2 / 3) - 1 : 1;
         N3p = N3;
-        zch = xpass_levels(N2, N3);
+        zch = xpass_levels(N1, N2, N3, three, int(sizeof(T)));
         nmodal = size_t(A1) * A2 * N3;
         nnodal = size_t(N1) * N2 * N3;
         SF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));

@@ -103,7 +103,8 @@ hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, cons
                        const T* zc, int N3p, hipStream_t s);
 
 bool tz_supported(int n);
-int xpass_levels(int N2, int N3);
+// z levels per x-pass block for this grid (see sf_kernels.hip)
+int xpass_levels(int N1, int N2, int N3, bool threeD, int precision);
 // twiddle table of an n-point LDS FFT (the layout its second pass reads)
 void fft_twiddles(int n, double* re, double* im);
 int max_rows_per_lane();

@@ -816,13 +816,54 @@ hipError_t launch_xpass(const XArgs<T>& a, int N1, int N2, bool threeD, hipStrea
     }
 }
 
-// levels per x-pass block: enough blocks to fill the chip several times over,
-// at most 32 levels so that the ring prologue (two extra levels) stays cheap
-int xpass_levels(int N2, int N3) {
-    int zc = (N2 * N3) / 1024;
-    if (zc > 32) zc = 32;
-    if (zc < 2) zc = 2;
-    return zc;
+// Levels per x-pass block.  Each block pays two extra ring levels, and the
+// last round of blocks may leave CUs idle; pick the height in [2, 48] that
+// minimises (rounds of resident blocks) x (levels per block + 2), given how
+// many blocks of this kernel the GPU keeps resident.
+template <typename T, int N1, bool THREED, bool LIN>
+static int xpass_resident(int ncu) {
+    constexpr int lds = ((THREED ? 6 : 3) + 3 * (LIN ? 4 : 2)) * x_ldp<N1>() * int(sizeof(cx<T>));
+    if constexpr (lds > 160 * 1024) {
+        return 0;
+    } else {
+        int per = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_xpass<T, N1, THREED, LIN>, BS, 0) != hipSuccess)
+            per = 1;
+        return (per < 1 ? 1 : per) * ncu;
+    }
+}
+
+int xpass_levels(int N1, int N2, int N3, bool threeD, int precision) {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    int slots = ncu;
+    switch (N1) {
+#define SF_CASE(n)                                                                                             \
+    case n:                                                                                                    \
+        slots = precision == 4 ? (threeD ? xpass_resident<float, n, true, false>(ncu)                          \
+                                         : xpass_resident<float, n, false, false>(ncu))                        \
+                               : (threeD ? xpass_resident<double, n, true, false>(ncu)                         \
+                                         : xpass_resident<double, n, false, false>(ncu));                      \
+        break;
+        SF_SIZES(SF_CASE)
+#undef SF_CASE
+        default:
+            break;
+    }
+    if (slots < 1) slots = ncu;
+    const int rows = threeD ? N2 : 1;
+    long best = -1;
+    int pick = 2;
+    for (int zc = 2; zc <= 48; ++zc) {
+        const long nblk = long((N3 + zc - 1) / zc) * rows;
+        const long rounds = (nblk + slots - 1) / slots;
+        const long cost = rounds * (zc + 2);
+        if (best < 0 || cost < best) {
+            best = cost;
+            pick = zc;
+        }
+    }
+    return pick;
 }
 
 void fft_twiddles(int n, double* re, double* im) {
