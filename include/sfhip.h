@@ -197,6 +197,37 @@ int sf_batch_step(sf_batch* b, int nsteps);
 int sf_batch_time_steps(sf_batch* b, int nsteps, double* total_ms);
 int sf_batch_info(const sf_batch* b, int* launches_per_step, int* xpass_groups, int* modal_groups);
 
+/* ---- learned preconditioner ----------------------------------------------
+ * No counterpart in the reference, whose Newton-GMRES is unpreconditioned
+ * (NewtonKrylov.h:138-235).  A small 1D convolutional network over the
+ * vertical profile of every horizontal mode (layers x Conv1D(kernel 5),
+ * `hidden` channels, ReLU), BF16 weights and activations with FP32
+ * accumulation on the matrix cores, trained on the GPU with Adam.  Applied as
+ * a right preconditioner:  out = in + s * net(in / s),  s = RMS(in), the
+ * identity until trained (the last layer starts at zero).  Field arrays hold
+ * u1, u2, u3, b (u2 ignored in 2D).  Training pairs (w, q) satisfy A q = w
+ * for the operator being preconditioned; each pair stores one sample per
+ * horizontal mode. */
+typedef struct sf_net sf_net;
+int sf_net_create(sf_ctx* ctx, int hidden, int layers, unsigned long long seed, sf_net** out);
+int sf_net_destroy(sf_net* n);
+const char* sf_net_last_error(const sf_net* n);
+int sf_net_apply(sf_net* n, sf_field* const* in, sf_field* const* out);
+int sf_net_add_pair(sf_net* n, sf_field* const* w, sf_field* const* q);
+int sf_net_clear_pairs(sf_net* n);
+int sf_net_samples(const sf_net* n, int* count);
+int sf_net_train(sf_net* n, int epochs, int batch, double lr, double* loss_first, double* loss_last);
+/* parameters: FP32 master copy, per layer l: W[5][C[l+1]][C[l]] then bias[C[l+1]] */
+int sf_net_param_count(const sf_net* n, size_t* count);
+int sf_net_get_params(sf_net* n, float* host);
+int sf_net_set_params(sf_net* n, const float* host);
+/* layers, padded channel counts C[0..layers] and profile length (N3) */
+int sf_net_geometry(const sf_net* n, int* layers, int* channels, int* z);
+/* raw tensors [profile][z][channel] (checks): forward of nprof profiles, and
+ * the mean-squared-error loss against target with its parameter gradient */
+int sf_net_forward_raw(sf_net* n, const float* in, int nprof, float* out);
+int sf_net_grad_raw(sf_net* n, const float* in, const float* target, int nprof, float* grad, double* loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,9 @@ EXPORTED = [
     "sf_energy", "sf_cfl", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
     "sf_batch_create", "sf_batch_destroy", "sf_batch_last_error", "sf_batch_step", "sf_batch_time_steps",
     "sf_batch_info",
+    "sf_net_create", "sf_net_destroy", "sf_net_last_error", "sf_net_apply", "sf_net_add_pair", "sf_net_clear_pairs",
+    "sf_net_samples", "sf_net_train", "sf_net_param_count", "sf_net_get_params", "sf_net_set_params",
+    "sf_net_geometry", "sf_net_forward_raw", "sf_net_grad_raw",
 ]
 KERNEL_CLASSES = ["yinv", "xpass", "yfwd", "modal"]
 
@@ -119,6 +122,21 @@ def lib():
         "sf_batch_time_steps": [vp, i, dp],
         "sf_batch_info": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                           ctypes.POINTER(ctypes.c_int)],
+        "sf_net_create": [vp, i, i, ctypes.c_ulonglong, vpp],
+        "sf_net_destroy": [vp],
+        "sf_net_apply": [vp, vpp, vpp],
+        "sf_net_add_pair": [vp, vpp, vpp],
+        "sf_net_clear_pairs": [vp],
+        "sf_net_samples": [vp, ctypes.POINTER(ctypes.c_int)],
+        "sf_net_train": [vp, i, i, d, dp, dp],
+        "sf_net_param_count": [vp, ctypes.POINTER(ctypes.c_size_t)],
+        "sf_net_get_params": [vp, ctypes.POINTER(ctypes.c_float)],
+        "sf_net_set_params": [vp, ctypes.POINTER(ctypes.c_float)],
+        "sf_net_geometry": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_int)],
+        "sf_net_forward_raw": [vp, ctypes.POINTER(ctypes.c_float), i, ctypes.POINTER(ctypes.c_float)],
+        "sf_net_grad_raw": [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), i,
+                            ctypes.POINTER(ctypes.c_float), dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -128,6 +146,8 @@ def lib():
     L.sf_last_error.restype = ctypes.c_char_p
     L.sf_batch_last_error.argtypes = [vp]
     L.sf_batch_last_error.restype = ctypes.c_char_p
+    L.sf_net_last_error.argtypes = [vp]
+    L.sf_net_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -398,6 +418,101 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib().sf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Net:
+    """Learned right preconditioner (sf_net_*): Conv1D layers over the vertical
+    profile of every horizontal mode, BF16 on the matrix cores, trained on the GPU.
+
+    apply(in_fields, out_fields) computes out = in + s * net(in / s), s = RMS(in);
+    the field lists hold u1, u2, u3, b (u2 may be None in 2D)."""
+
+    def __init__(self, solver, hidden=64, layers=4, seed=0):
+        L = lib()
+        self.solver = solver
+        h = ctypes.c_void_p()
+        rc = L.sf_net_create(solver.h, hidden, layers, seed, ctypes.byref(h))
+        if rc != 0:
+            raise SFError(rc, L.sf_net_last_error(None).decode())
+        self.h = h
+        nl = ctypes.c_int()
+        ch = (ctypes.c_int * 16)()
+        z = ctypes.c_int()
+        self._chk(L.sf_net_geometry(self.h, ctypes.byref(nl), ch, ctypes.byref(z)))
+        self.layers = nl.value
+        self.channels = [ch[k] for k in range(nl.value + 1)]
+        self.Z = z.value
+        n = ctypes.c_size_t()
+        self._chk(L.sf_net_param_count(self.h, ctypes.byref(n)))
+        self.nparam = n.value
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise SFError(rc, lib().sf_net_last_error(self.h).decode())
+
+    @staticmethod
+    def _handles(fields):
+        arr = (ctypes.c_void_p * 4)(*[(f.h.value if f is not None else None) for f in fields])
+        return arr
+
+    def apply(self, inp, out):
+        self._chk(lib().sf_net_apply(self.h, self._handles(inp), self._handles(out)))
+
+    def add_pair(self, w, q):
+        self._chk(lib().sf_net_add_pair(self.h, self._handles(w), self._handles(q)))
+
+    def clear_pairs(self):
+        self._chk(lib().sf_net_clear_pairs(self.h))
+
+    def samples(self):
+        c = ctypes.c_int()
+        self._chk(lib().sf_net_samples(self.h, ctypes.byref(c)))
+        return c.value
+
+    def train(self, epochs, batch, lr):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._chk(lib().sf_net_train(self.h, epochs, batch, lr, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def get_params(self):
+        p = np.zeros(self.nparam, dtype=np.float32)
+        self._chk(lib().sf_net_get_params(self.h, _fp(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        self._chk(lib().sf_net_set_params(self.h, _fp(p)))
+
+    def forward_raw(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        nprof = x.shape[0]
+        out = np.zeros((nprof, self.Z, self.channels[-1]), dtype=np.float32)
+        self._chk(lib().sf_net_forward_raw(self.h, _fp(x), nprof, _fp(out)))
+        return out
+
+    def grad_raw(self, x, target):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(self.nparam, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._chk(lib().sf_net_grad_raw(self.h, _fp(x), _fp(t), x.shape[0], _fp(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sf_net_destroy(self.h)
             self.h = None
 
     def __del__(self):

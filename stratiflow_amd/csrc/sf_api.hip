@@ -8,25 +8,10 @@
 #include <vector>
 
 #include "../../include/sfhip.h"
+#include "sf_internal.h"
 #include "sf_kernels.h"
 
 namespace sf {
-
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define SF_HIP(expr)                                                                                    \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess) throw ::sf::Error(SF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-#define SF_REQUIRE(cond, code, msg)                  \
-    do {                                             \
-        if (!(cond)) throw ::sf::Error(code, msg);   \
-    } while (0)
 
 // ---------------------------------------------------------------------------
 // Vertical grid (Field.cpp:3-31, 62-83) in double.  z[0] and DY[0], which
@@ -75,22 +60,6 @@ struct FieldBase {
 };
 
 }  // namespace sf
-
-struct sf_field {
-    sf_ctx* ctx;
-    int kind;
-    int bc;
-    size_t n;       // scalars of type T (complex counts twice)
-    void* d;        // device pointer
-    bool owned;     // false for solver-owned fields
-};
-
-struct sf_ctx {
-    int precision;
-    std::string err;
-    std::shared_ptr<void> impl;  // sf::Ctx<T>
-    void* raw;
-};
 
 struct sf_batch {
     int precision;
@@ -1018,6 +987,23 @@ static int batch_call(sf_batch* b, Fn fn) {
         b->err = e.what();
         return SF_ERR_STATE;
     }
+}
+
+sf::CtxInfo sf_ctx_info(const sf_ctx* c) {
+    sf::CtxInfo info{};
+    info.precision = c->precision;
+    with_ctx(const_cast<sf_ctx*>(c), [&](auto& C) {
+        info.N1 = C.N1;
+        info.N2 = C.N2;
+        info.N3 = C.N3;
+        info.A1 = C.A1;
+        info.A2 = C.A2;
+        info.three = C.three;
+        info.stream = C.stream;
+        info.kx = C.kx;
+        info.ky = C.ky;
+    });
+    return info;
 }
 
 extern "C" {
