@@ -82,7 +82,7 @@ public:
             // the Krylov space is rebuilt for every Newton step (NewtonKrylov.h:88-93)
             vectorsToReuse = 0;
             H.setZero();
-            GMRES(rhs, dx, gmresTolerance, Delta);
+            SolveLinear(rhs, dx, gmresTolerance, Delta);
             x += dx;
             // repeated, as relative precision can make it better (NewtonKrylov.h:101-104)
             EnforceConstraints(x);
@@ -104,6 +104,7 @@ public:
 
     // what the last solve did
     int newtonSteps = 0;
+    long matVecs = 0;  // operator applications (time integrations) over all solves
     int lastKrylovDimension = 0;
     double lastGmresResidual = 0;
     std::vector<double> residualHistory;
@@ -135,6 +136,8 @@ public:
                     q[k] = EvalDerivative(q[k - 1]);
                 }
                 q[k] *= stratifloat(-1.0);  // factor of -1 for Newton iteration
+                ++matVecs;
+                OnMatVec(flexible ? Z[k - 1] : q[k - 1], q[k]);
                 // modified Gram-Schmidt, FP64 coefficients
                 for (int j = 0; j < k; j++) {
                     H(j, k - 1) = sfkrylov::dot(q[j], q[k]);
@@ -171,6 +174,16 @@ protected:
     // right preconditioner M^-1; return true from HasPreconditioner to use it
     virtual bool HasPreconditioner() const { return false; }
     virtual VectorType Precondition(const VectorType& v) { return v; }
+    // the linear solve of each Newton step (GMRES, NewtonKrylov.h:96)
+    virtual void SolveLinear(const VectorType& rhs, VectorType& dx, double epsilon, double Delta_) {
+        vectorsToReuse = 0;
+        GMRES(rhs, dx, epsilon, Delta_);
+    }
+    // every operator application of GMRES: Ax = A x with A = -dF/dx
+    virtual void OnMatVec(const VectorType& x, const VectorType& Ax) {
+        (void)x;
+        (void)Ax;
+    }
 
     stratifloat T = 11;  // time interval for integration (NewtonKrylov.h:113)
 

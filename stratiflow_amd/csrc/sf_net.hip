@@ -10,7 +10,9 @@
 //     M^-1 v = v + s * net(v / s),   s = RMS(v),
 // which is the identity while the last layer is zero (its initial value) and
 // positively homogeneous in v.  Training pairs (w, q) with A q = w (w = A q is
-// what every Krylov step of GMRES computes) teach net(w/s) ~ (q - w)/s.
+// what every Krylov step of GMRES computes) teach net(w/s) ~ q/RMS(q) - w/s:
+// M^-1 w then points along A^-1 w, which is all a (flexible) GMRES direction
+// needs, and the targets stay O(1) however small the eigenvalues of A are.
 //
 // Tensors are [position][channel] with channels fastest, position p = profile
 // * N3 + z; channel counts are padded to multiples of 32 (the MFMA K step).
@@ -296,7 +298,9 @@ __global__ __launch_bounds__(NB) void k_pack(PackArgs A, bf16* __restrict__ in) 
     }
 }
 
-// training target: (q - w) / s on the real output channels
+// training target: q / RMS(q) - w / RMS(w) on the real output channels (the
+// direction of A^-1 w at the scale of w, minus w; W.sinv and Q.sinv hold the
+// two inverse RMS values)
 template <typename T>
 __global__ __launch_bounds__(NB) void k_pack_target(PackArgs W, PackArgs Q, int Cout, float* __restrict__ tgt) {
     const size_t P = size_t(W.A1) * W.A2 * W.N3;
@@ -312,7 +316,7 @@ __global__ __launch_bounds__(NB) void k_pack_target(PackArgs W, PackArgs Q, int 
             const size_t o = sidx(ja, z, j1, W.N3, W.A1);
             const cx<T> w = static_cast<const cx<T>*>(W.f[c >> 1])[o];
             const cx<T> q = static_cast<const cx<T>*>(Q.f[c >> 1])[o];
-            v = float(double((c & 1) ? (q.y - w.y) : (q.x - w.x)) * W.sinv);
+            v = float((c & 1) ? double(q.y) * Q.sinv - double(w.y) * W.sinv : double(q.x) * Q.sinv - double(w.x) * W.sinv);
         }
         tgt[e] = v;
     }
@@ -685,10 +689,10 @@ struct Net {
     void add_pair(sf_field* const* wf, sf_field* const* qf) {
         const int np = int(profiles());
         grow_set(np);
-        const double s = rms<T>(wf);
-        if (!(s > 0)) return;
+        const double s = rms<T>(wf), sq = rms<T>(qf);
+        if (!(s > 0) || !(sq > 0)) return;
         PackArgs W = pack_args<T>(wf, 1.0 / s);
-        PackArgs Q = pack_args<T>(qf, 1.0 / s);
+        PackArgs Q = pack_args<T>(qf, 1.0 / sq);
         bf16* di = set_in + size_t(set_n) * Z * C[0];
         float* dt = set_t + size_t(set_n) * Z * C[L];
         const size_t P = size_t(np) * Z;

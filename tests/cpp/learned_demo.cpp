@@ -1,0 +1,112 @@
+// Trains the learned preconditioner (include/stratiflow/LearnedPreconditioner.h)
+// on the operator applications of plain GMRES solves, then solves held-out
+// problems with and without it.  FP32 time stepping (the facade's default
+// precision), FP64 Krylov algebra.  Prints, one per line:
+//   train <problem> <gmres iterations> <seconds>
+//   loss <first epoch> <last epoch> <samples> <train seconds>
+//   test <problem> <plain matvecs> <plain s> <learned matvecs> <learned s> <plain res> <learned res>
+//        <rel diff of the Newton updates> <fallbacks so far>
+//   hist <problem> <GMRES residual history of the plain solve>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "stratiflow/LearnedPreconditioner.h"
+#include "stratiflow/NewtonKrylov.h"
+
+// F(x) = Phi(x) - x, Phi = nsteps fixed-dt steps (FixedEvolve start-up)
+class FixedNewton : public NewtonKrylov<StateVector> {
+public:
+    stratifloat dt = stratifloat(0.02);
+    int nsteps = 100;
+
+protected:
+    StateVector EvalFunction(const StateVector& at) override {
+        StateVector s = at;
+        std::vector<StateVector> traj;
+        s.FixedEvolve(dt, nsteps + 1, traj);
+        StateVector r = traj[nsteps];
+        r -= at;
+        return r;
+    }
+};
+
+typedef WithLearnedPreconditioner<FixedNewton> Learned;
+
+static double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static StateVector problem(unsigned seed) {
+    SeedRandom(seed);
+    StateVector x;
+    x.ExciteLowWavenumbers(stratifloat(1e-3 * (1.0 + 0.5 * (seed % 3))));
+    return x;
+}
+
+int main(int argc, char** argv) {
+    const int ntrain = argc > 1 ? std::atoi(argv[1]) : 6;
+    const int ntest = argc > 2 ? std::atoi(argv[2]) : 3;
+    const int K = argc > 3 ? std::atoi(argv[3]) : 61;
+    const double tol = argc > 4 ? std::atof(argv[4]) : 1e-3;
+    const int epochs = argc > 5 ? std::atoi(argv[5]) : 150;
+    gridParams = {64, 1, 64, Dimensionality::TwoDimensional};
+    flowParams.L1 = stratifloat(2 * 3.14159265358979 / 0.44);
+    flowParams.L2 = 1;
+    flowParams.L3 = 10;
+    flowParams.Re = argc > 6 ? std::atof(argv[6]) : 100;
+    flowParams.Ri = stratifloat(0.1);
+    flowParams.Pr = 1;
+    StateVector::ResetForParams();
+    StateVector::solver().SetBackground(InitialU);
+
+    LearnedPreconditioner pre(64, 4, 1);
+    Learned nk(pre);
+    nk.verbose = false;
+    nk.SetKrylovDimension(K);
+    nk.gmresTolerance = tol;
+    nk.maxNewtonSteps = 1;
+
+    // harvest (A x, x) from plain solves
+    nk.harvest = true;
+    nk.enabled = false;
+    for (int i = 0; i < ntrain; ++i) {
+        StateVector x = problem(100 + i);
+        const double t0 = now();
+        nk.Run(x);
+        printf("train %d %d %.4f\n", i, nk.lastKrylovDimension, now() - t0);
+    }
+    nk.harvest = false;
+    double l0 = 0, l1 = 0;
+    const double t0 = now();
+    pre.Train(epochs, 64, 1e-3, &l0, &l1);
+    printf("loss %.6e %.6e %d %.3f\n", l0, l1, pre.Samples(), now() - t0);
+
+    for (int i = 0; i < ntest; ++i) {
+        const StateVector x0 = problem(1000 + i);
+        StateVector xp = x0, xl = x0;
+        nk.enabled = false;
+        long m0 = nk.matVecs;
+        double t = now();
+        nk.Run(xp);
+        const double tp = now() - t;
+        const long mp = nk.matVecs - m0;
+        const double rp = nk.lastGmresResidual;
+        printf("hist %d", i);
+        for (double r : nk.gmresResidualHistory) printf(" %.3e", r);
+        printf("\n");
+        nk.enabled = true;
+        m0 = nk.matVecs;
+        t = now();
+        nk.Run(xl);
+        const double tl = now() - t;
+        const long ml = nk.matVecs - m0;
+        const double rl = nk.lastGmresResidual;
+        StateVector dp = xp - x0, dl = xl - xp;
+        printf("test %d %ld %.4f %ld %.4f %.3e %.3e %.3e %d\n", i, mp, tp, ml, tl, rp, rl,
+               sfkrylov::norm(dl) / sfkrylov::norm(dp), nk.fallbacks);
+    }
+    return 0;
+}

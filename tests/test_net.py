@@ -108,7 +108,7 @@ def test_training_lowers_loss_and_identity_until_trained():
     net.apply(fin, outs)
     for f, o in zip(fin, outs):
         assert np.array_equal(f.download(), o.download())
-    # pairs of a fixed linear map applied per mode: q = 0.5 w  (target q - w = -0.5 w)
+    # pairs of a fixed linear map applied per mode: q = 0.5 w (direction target 0)
     rng = np.random.default_rng(2)
     for k in range(6):
         w = [s.field(sf.MODAL, sf.DIRICHLET if n == "u3" else sf.NEUMANN) for n in ("u1", "u2", "u3", "b")]
