@@ -109,6 +109,51 @@ def cpu_baseline(U, S, pert, max_seconds):
                       "FFT, banded vertical operators), setup excluded; %s" % (nstages, cpu)}
 
 
+def ensemble(sf, ic, nruns=64, steps=500, seed=20261202):
+    """Batched 2D ensemble: nruns seeded runs with uneven grids and parameters,
+    advanced together (sf_batch_*) and, for comparison, one after the other."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    specs = [(int(rng.choice([64, 96, 128, 192, 256])), int(rng.choice([96, 128, 192, 256, 384])),
+              float(rng.uniform(500, 4000)), float(rng.uniform(0.05, 0.25))) for _ in range(nruns)]
+
+    def make(k, N1, N3, Re, Ri):
+        L1 = 2 * np.pi / 0.44
+        s = sf.IMEXRK(N1, 1, N3, sf.TWO_D, L1, 1.0, 10.0, Re, Ri, 1.0, precision=4)
+        z, zf = s.vertical_points()
+        s.SetBackground(ic.tanh_background(zf))
+        for n, a in ic.perturbation(N1, 1, N3, sf.TWO_D, z, zf, seed + k, amplitude=1e-3).items():
+            s.set(n, a)
+        s.FilterAll()
+        s.PopulateNodalVariables()
+        s.RemoveDivergence(0.0)
+        s.deltaT = 0.01
+        return s
+
+    runs = [make(k, *sp) for k, sp in enumerate(specs)]
+    b = sf.Batch(runs)
+    b.step(1)  # first step (nodal snapshots) and graph capture
+    ms_batch = b.time_steps(steps)
+    seq = [make(k, *sp) for k, sp in enumerate(specs)]
+    for r in seq:
+        r.TimeStep(1)
+        r.synchronize()
+    t0 = time.perf_counter()
+    for r in seq:
+        r.TimeStep(steps)
+    for r in seq:
+        r.synchronize()
+    ms_seq = (time.perf_counter() - t0) * 1e3
+    info = b.info()
+    out = {"runs": nruns, "steps": steps, "grids": "N1 in {64,96,128,192,256}, N3 in {96,128,192,256,384}",
+           "batched_run_steps_per_s": nruns * steps / (ms_batch * 1e-3),
+           "sequential_run_steps_per_s": nruns * steps / (ms_seq * 1e-3),
+           "speedup": ms_seq / ms_batch, "launches_per_step": info["launches_per_step"],
+           "xpass_groups": info["xpass_groups"], "modal_groups": info["modal_groups"]}
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +162,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=40.0, help="budget for the CPU baseline sample")
     ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_384x192x384_fp32.json"))
+    ap.add_argument("--ensemble", action="store_true",
+                    help="also time the batched 64-run 2D ensemble against sequential runs")
     ap.add_argument("--calibration-copies", type=int, default=0,
                     help="device-to-device copies of one modal field (known bytes) after the timed region, "
                          "to scale FETCH_SIZE/WRITE_SIZE counters")
@@ -198,6 +245,8 @@ def main():
         "energy": {"KE": ke, "PE": pe},
         "cpu_baseline": None,
     }
+    if args.ensemble:
+        out["ensemble"] = ensemble(sf, ic)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(U, S, pert, args.cpu_seconds)
     print(json.dumps(out), flush=True)
