@@ -150,6 +150,18 @@ int sf_step_linear(sf_ctx* ctx, int nsteps);
  * SF_B_TOT from the given modal fields (u2 may be NULL in 2D) */
 int sf_set_linear_background(sf_ctx* ctx, const sf_field* u1, const sf_field* u2, const sf_field* u3,
                              const sf_field* b);
+/* IMEXRK::TimeStepAdjoint (IMEXRK.h:86-126, with UpdateAdjointVariables
+ * IMEXRK.h:461-500 and BuildRHSAdjoint IMEXRK.cpp:260-379): one backward
+ * step of the adjoint state held in SF_U1..SF_B, with the direct state
+ * below[] = {u1, u2, u3, b} at the end and above[] at the start of the step
+ * (reference argument order u1Below..bBelow, u1Above..bAbove).  The direct
+ * states are modal fields allocated with sf_field_alloc (u2 may be NULL in
+ * 2D).  Overwrites SF_U1_TOT..SF_B_TOT with the interpolated direct state, as
+ * the reference overwrites u1_tot..b_tot.  Call after the set-up of
+ * StateVector::AdjointEvolve (StateVector.cpp:158-172): FilterAll,
+ * PopulateNodalVariables, RemoveDivergence(0), zero pressure,
+ * PopulateNodalVariables (PrepareRunAdjoint, IMEXRK.h:183-199). */
+int sf_step_adjoint(sf_ctx* ctx, const sf_field* const below[4], const sf_field* const above[4]);
 /* one RK stage k of TimeStep (0..2) or TimeStepLinear (linear != 0) */
 int sf_stage(sf_ctx* ctx, int k, int linear);
 /* IMEXRK::RemoveDivergence (IMEXRK.cpp:43-73) */

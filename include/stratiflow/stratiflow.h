@@ -551,6 +551,28 @@ public:
         sync_dt();
         sfacade::check(sf_step_linear(c(), 1), c());
     }
+    // IMEXRK.h:86-126: one backward step of the adjoint state in u1..b with the
+    // direct state at the end (Below) and start (Above) of the step
+    void TimeStepAdjoint(const NeumannModal& u1Below, const NeumannModal& u2Below, const DirichletModal& u3Below,
+                         const NeumannModal& bBelow, const NeumannModal& u1Above, const NeumannModal& u2Above,
+                         const DirichletModal& u3Above, const NeumannModal& bAbove) {
+        sync_dt();
+        const bool three = gridParams.dimensionality == Dimensionality::ThreeDimensional;
+        const sf_field* below[4] = {u1Below.dev(false), three ? u2Below.dev(false) : nullptr, u3Below.dev(false),
+                                    bBelow.dev(false)};
+        const sf_field* above[4] = {u1Above.dev(false), three ? u2Above.dev(false) : nullptr, u3Above.dev(false),
+                                    bAbove.dev(false)};
+        sfacade::check(sf_step_adjoint(c(), below, above), c());
+        U1.dev(true);
+        U2.dev(true);
+        U3.dev(true);
+        B.dev(true);
+    }
+    // IMEXRK.h:183-199 without the image folders
+    void PrepareRunAdjoint() {
+        p.Zero();
+        PopulateNodalVariables();
+    }
     void RemoveDivergence(stratifloat pressureMultiplier = 1.0) {  // IMEXRK.cpp:43-73
         flush_state();
         sfacade::check(sf_remove_divergence(c(), pressureMultiplier), c());
@@ -758,6 +780,26 @@ public:
             CopyFromSolver(result[step]);
             solver().TimeStep();
         }
+    }
+
+    // StateVector.cpp:158-187: steps backward steps of the adjoint from this
+    // state, with intermediate[0..steps] the direct states of the interval
+    void AdjointEvolve(stratifloat deltaT, int steps, const std::vector<StateVector>& intermediate,
+                       StateVector& result) const {
+        CopyToSolver();
+        solver().SetBackground(InitialU);
+        solver().FilterAll();
+        solver().PopulateNodalVariables();
+        solver().RemoveDivergence(0.0f);
+        solver().PrepareRunAdjoint();
+        solver().deltaT = deltaT;
+        solver().UpdateForTimestep();
+        for (int step = 0; step < steps; ++step) {
+            const StateVector& below = intermediate[steps - 1 - step];
+            const StateVector& above = intermediate[steps - step];
+            solver().TimeStepAdjoint(below.u1, below.u2, below.u3, below.b, above.u1, above.u2, above.u3, above.b);
+        }
+        CopyFromSolver(result);
     }
 
     // StateVector.cpp:3-83 without plotting or snapshots

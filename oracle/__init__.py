@@ -70,6 +70,8 @@ def lib():
         L.sfo_tri_solve.argtypes = [vp, i, i, i, i, dp, dp, dp, dp, dp]
         L.sfo_apply_op.argtypes = [vp, i, dp, dp]
         L.sfo_apply_op.restype = i
+        L.sfo_prepare_adjoint.argtypes = [vp]
+        L.sfo_adjoint.argtypes = [vp, i, ctypes.POINTER(dp), ctypes.POINTER(dp), d]
         _lib = L
     return _lib
 
@@ -161,6 +163,39 @@ class Oracle:
 
     def step_linear(self, n=1):
         lib().sfo_step_linear(self.h, n)
+
+    def prepare_adjoint(self):
+        """PrepareRunAdjoint (IMEXRK.h:183-199): zero pressure, populate nodal fields."""
+        lib().sfo_prepare_adjoint(self.h)
+
+    def step_adjoint(self, below, above):
+        """One TimeStepAdjoint (IMEXRK.h:86-126).  below/above: dicts u1, u2, u3, b of
+        modal arrays (reference layout); a missing field is zero."""
+        self._adjoint(0, below, above, 0.0)
+
+    def rhs_adjoint(self, below, above, frac):
+        """Explicit terms of BuildRHSAdjoint (IMEXRK.cpp:260-379) into r1..rB, with the
+        direct state interpolated at frac and the nodal fields as they are."""
+        self._adjoint(1, below, above, frac)
+
+    def rhs_linear(self):
+        """Explicit terms of BuildRHSLinear (IMEXRK.cpp:195-258) into r1..rB."""
+        self._adjoint(2, {}, {}, 0.0)
+
+    def _adjoint(self, mode, below, above, frac):
+        keep = []
+
+        def ptrs(st):
+            arr = (ctypes.POINTER(ctypes.c_double) * 4)()
+            for i, n in enumerate(("u1", "u2", "u3", "b")):
+                if n in st and st[n] is not None:
+                    a = np.ascontiguousarray(np.asarray(st[n], dtype=np.complex128).reshape(self.modal_shape))
+                    keep.append(a)
+                    arr[i] = _p(a.view(np.float64))
+                else:
+                    arr[i] = ctypes.POINTER(ctypes.c_double)()
+            return arr
+        lib().sfo_adjoint(self.h, mode, ptrs(below), ptrs(above), frac)
 
     def energy(self):
         ke, pe = ctypes.c_double(), ctypes.c_double()

@@ -17,6 +17,7 @@
 //   * IMEXRK::TimeStep and its stages     IMEXRK.cpp:8-24, 43-193; IMEXRK.h:37-81, 128-162, 502-594
 //   * InnerProd / KE / PE                 Integration.h:5-35, 117-146; IMEXRK.h:395-411
 //   * TimeStepLinear / BuildRHSLinear     IMEXRK.cpp:26-41, 195-258; IMEXRK.h:334-352
+//   * TimeStepAdjoint / BuildRHSAdjoint   IMEXRK.h:86-126, 183-199, 461-500; IMEXRK.cpp:260-379
 //
 // Deliberate differences from the reference (all tolerance-level):
 //   * grid and operator coefficients are computed in double and rounded to T;
@@ -231,6 +232,7 @@ public:
     std::vector<C> u1t, u2t, u3t, bt;            // linearisation state (IMEXRK.h:614-615)
     std::vector<C> neumannTemp, dirichletTemp, divergence, q;
     std::vector<T> U1, U2, U3, B, U1_tot, U2_tot, U3_tot, B_tot, nnTemp, prod;
+    std::vector<T> u1F, u2F, u3F, bF, nnTemp2, ndTemp, ndTemp2;  // IMEXRK.h:625-647
     std::vector<T> Ubar;  // Neumann1D U_ (IMEXRK.h:609)
     std::vector<T> S;     // dB/dz profile at Neumann points (1 in the reference)
     double deltaT = 0.01;
@@ -244,7 +246,8 @@ public:
                 &u1t, &u2t, &u3t, &bt, &neumannTemp, &dirichletTemp, &divergence, &q};
     }
     std::vector<std::vector<T>*> nodal_fields() {
-        return {&U1, &U2, &U3, &B, &U1_tot, &U2_tot, &U3_tot, &B_tot, &nnTemp, &prod};
+        return {&U1, &U2, &U3, &B, &U1_tot, &U2_tot, &U3_tot, &B_tot, &nnTemp, &prod,
+                &u1F, &u2F, &u3F, &bF, &nnTemp2, &ndTemp, &ndTemp2};
     }
 
     size_t mi(int j1, int j2, int j3) const { return (size_t(M1) * j2 + j1) * N3 + j3; }
@@ -920,6 +923,256 @@ public:
     void time_step() { for (int k = 0; k < 3; ++k) stage(k, false); }        // IMEXRK.cpp:8-24
     void time_step_linear() { for (int k = 0; k < 3; ++k) stage(k, true); }  // IMEXRK.cpp:26-41
 
+    // ---- adjoint stepping ----
+    // r += f(stack) over active stacks (BuildRHSAdjoint adds its terms)
+    template <typename F>
+    void modal_add(std::vector<C>& r, F f) {
+        per_active([&](int j1, int j2) {
+            std::vector<C> t(N3);
+            f(j1, j2, mi(j1, j2, 0), t.data());
+            C* rr = &r[mi(j1, j2, 0)];
+            for (int j = 0; j < N3; ++j) rr[j] += t[j];
+        });
+    }
+    // out = ddx(in), ddy(in) (Stratiflow.h:52-66) or ddz(in) (Stratiflow.h:68-81,
+    // Neumann in -> Dirichlet out and the reverse) of a modal field
+    void modal_deriv(int axis, int bc_in, const std::vector<C>& in, std::vector<C>& out) {
+        per_active([&](int j1, int j2) {
+            const size_t o = mi(j1, j2, 0);
+            if (axis == 2) {
+                if (bc_in == NEUMANN) op_dz_ND(&in[o], &out[o]);
+                else op_dz_DN(&in[o], &out[o]);
+                return;
+            }
+            const C ik(0, axis == 0 ? kx[j1] : ky[j2]);
+            for (int j = 0; j < N3; ++j) out[o + j] = ik * in[o + j];
+        });
+    }
+
+    // UpdateAdjointVariables, IMEXRK.h:461-500 (the J integrand is a
+    // diagnostic and is not formed)
+    void update_adjoint_variables() {
+        to_nodal(u1t, U1_tot);
+        to_nodal(u2t, U2_tot);
+        to_nodal(u3t, U3_tot);
+        to_nodal(bt, B_tot);
+        std::vector<T> bAve(N3), wAve(N3);  // HorizontalAverage, Integration.h:37-42
+        for (int j = 0; j < N3; ++j) {
+            bAve[j] = bt[mi(0, 0, j)].real();
+            wAve[j] = u3t[mi(0, 0, j)].real();
+        }
+        const T K = T(2);
+        per_nodal_stack([&](size_t o) {
+            for (int j = 0; j < N3; ++j) {
+                u3F[o + j] = (T(1) / K) * (B_tot[o + j] + T(-1) * bAve[j]);
+                bF[o + j] = (T(1) / K) * (U3_tot[o + j] + T(-1) * wAve[j]);
+                u1F[o + j] = T(0);
+                u2F[o + j] = T(0);
+            }
+        });
+    }
+
+    // BuildRHSAdjoint, IMEXRK.cpp:260-379
+    void build_rhs_adjoint() {
+        const T ri = T(Ri);
+        per_nodal_stack([&](size_t o) {
+            std::vector<T> t(N3);
+            op_idir(&U3[o], t.data());
+            for (int j = 0; j < N3; ++j) bF[o + j] += ri * t[j];
+        });
+        // advection of the adjoint fields by the direct flow
+        prod_nn(U1, U1_tot, neumannTemp);
+        modal_add(r1, [&](int j1, int, size_t o, C* t) {
+            const C ikx(0, kx[j1]);
+            for (int j = 0; j < N3; ++j) t[j] = ikx * neumannTemp[o + j];
+        });
+        prod_dbar(U1, U3_tot, neumannTemp);
+        modal_add(r1, [&](int, int, size_t o, C* t) {
+            for (int j = 0; j < N3; ++j) t[j] = neumannTemp[o + j];
+        });
+        prod_tilde(U1_tot, U3, dirichletTemp);
+        prod_dd(U3, U3_tot, neumannTemp);
+        modal_add(r3, [&](int j1, int, size_t o, C* t) {
+            const C ikx(0, kx[j1]);
+            std::vector<C> dz(N3);
+            op_dz_ND(&neumannTemp[o], dz.data());
+            for (int j = 0; j < N3; ++j) t[j] = ikx * dirichletTemp[o + j] + dz[j];
+        });
+        if (third) {
+            prod_nn(U2, U2_tot, neumannTemp);
+            modal_add(r2, [&](int, int j2, size_t o, C* t) {
+                const C iky(0, ky[j2]);
+                for (int j = 0; j < N3; ++j) t[j] = iky * neumannTemp[o + j];
+            });
+            prod_dbar(U2, U3_tot, neumannTemp);
+            modal_add(r2, [&](int, int, size_t o, C* t) {
+                for (int j = 0; j < N3; ++j) t[j] = neumannTemp[o + j];
+            });
+            prod_tilde(U2_tot, U3, dirichletTemp);
+            modal_add(r3, [&](int, int j2, size_t o, C* t) {
+                const C iky(0, ky[j2]);
+                for (int j = 0; j < N3; ++j) t[j] = iky * dirichletTemp[o + j];
+            });
+            prod_nn(U1, U2_tot, neumannTemp);
+            modal_add(r1, [&](int, int j2, size_t o, C* t) {
+                const C iky(0, ky[j2]);
+                for (int j = 0; j < N3; ++j) t[j] = iky * neumannTemp[o + j];
+            });
+            prod_nn(U2, U1_tot, neumannTemp);
+            modal_add(r2, [&](int j1, int, size_t o, C* t) {
+                const C ikx(0, kx[j1]);
+                for (int j = 0; j < N3; ++j) t[j] = ikx * neumannTemp[o + j];
+            });
+        }
+        prod_dbar(B, U3_tot, neumannTemp);
+        modal_add(rB, [&](int, int, size_t o, C* t) {
+            for (int j = 0; j < N3; ++j) t[j] = neumannTemp[o + j];
+        });
+        if (third) {
+            prod_nn(B, U2_tot, neumannTemp);
+            modal_add(rB, [&](int, int j2, size_t o, C* t) {
+                const C iky(0, ky[j2]);
+                for (int j = 0; j < N3; ++j) t[j] = iky * neumannTemp[o + j];
+            });
+        }
+        prod_nn(B, U1_tot, neumannTemp);
+        modal_add(rB, [&](int j1, int, size_t o, C* t) {
+            const C ikx(0, kx[j1]);
+            for (int j = 0; j < N3; ++j) t[j] = ikx * neumannTemp[o + j];
+        });
+
+        // extra adjoint terms: gradients of the direct flow acting on the adjoint
+        // horizontal derivatives (u1Forcing, and u2Forcing with a third dimension)
+        for (int axis = 0; axis < (third ? 2 : 1); ++axis) {
+            std::vector<T>& F = axis == 0 ? u1F : u2F;
+            modal_deriv(axis, NEUMANN, u1t, neumannTemp);
+            to_nodal(neumannTemp, nnTemp);
+            per_nodal_stack([&](size_t o) {
+                for (int j = 0; j < N3; ++j) nnTemp2[o + j] = nnTemp[o + j] * U1[o + j];
+            });
+            if (third) {
+                modal_deriv(axis, NEUMANN, u2t, neumannTemp);
+                to_nodal(neumannTemp, nnTemp);
+                per_nodal_stack([&](size_t o) {
+                    for (int j = 0; j < N3; ++j) nnTemp2[o + j] += nnTemp[o + j] * U2[o + j];
+                });
+            }
+            modal_deriv(axis, DIRICHLET, u3t, dirichletTemp);
+            to_nodal(dirichletTemp, ndTemp);
+            per_nodal_stack([&](size_t o) {
+                std::vector<T> pr(N3), ip(N3);
+                for (int j = 0; j < N3; ++j) pr[j] = ndTemp[o + j] * U3[o + j];
+                op_idir(pr.data(), ip.data());
+                for (int j = 0; j < N3; ++j) F[o + j] -= nnTemp2[o + j] + ip[j];
+            });
+        }
+        // vertical derivatives (u3Forcing)
+        modal_deriv(2, NEUMANN, u1t, dirichletTemp);
+        to_nodal(dirichletTemp, ndTemp);
+        per_nodal_stack([&](size_t o) {
+            std::vector<T> iu(N3);
+            op_ifull(&U1[o], iu.data());
+            for (int j = 0; j < N3; ++j) ndTemp2[o + j] = ndTemp[o + j] * iu[j];
+        });
+        if (third) {
+            modal_deriv(2, NEUMANN, u2t, dirichletTemp);
+            to_nodal(dirichletTemp, ndTemp);
+            per_nodal_stack([&](size_t o) {
+                std::vector<T> iu(N3);
+                op_ifull(&U2[o], iu.data());
+                for (int j = 0; j < N3; ++j) ndTemp2[o + j] += ndTemp[o + j] * iu[j];
+            });
+        }
+        modal_deriv(2, DIRICHLET, u3t, neumannTemp);
+        to_nodal(neumannTemp, nnTemp);
+        per_nodal_stack([&](size_t o) {
+            std::vector<T> in(N3);
+            op_ifull(&nnTemp[o], in.data());
+            for (int j = 0; j < N3; ++j) u3F[o + j] -= ndTemp2[o + j] + in[j] * U3[o + j];
+        });
+        // buoyancy gradients
+        modal_deriv(0, NEUMANN, bt, neumannTemp);
+        to_nodal(neumannTemp, nnTemp);
+        per_nodal_stack([&](size_t o) {
+            for (int j = 0; j < N3; ++j) u1F[o + j] -= nnTemp[o + j] * B[o + j];
+        });
+        if (third) {
+            modal_deriv(1, NEUMANN, bt, neumannTemp);
+            to_nodal(neumannTemp, nnTemp);
+            per_nodal_stack([&](size_t o) {
+                for (int j = 0; j < N3; ++j) u2F[o + j] -= nnTemp[o + j] * B[o + j];
+            });
+        }
+        modal_deriv(2, NEUMANN, bt, dirichletTemp);
+        to_nodal(dirichletTemp, ndTemp);
+        per_nodal_stack([&](size_t o) {
+            std::vector<T> ib(N3);
+            op_ifull(&B[o], ib.data());
+            for (int j = 0; j < N3; ++j) u3F[o + j] -= ndTemp[o + j] * ib[j];
+        });
+        // all forcing terms
+        auto add_modal = [&](std::vector<C>& r, const std::vector<T>& F, std::vector<C>& tmp) {
+            to_modal(F, tmp);
+            modal_add(r, [&](int, int, size_t o, C* t) {
+                for (int j = 0; j < N3; ++j) t[j] = tmp[o + j];
+            });
+        };
+        add_modal(r1, u1F, neumannTemp);
+        if (third) add_modal(r2, u2F, neumannTemp);
+        add_modal(r3, u3F, dirichletTemp);
+        add_modal(rB, bF, neumannTemp);
+    }
+
+    // the direct state at fraction f of the backward step, with the background
+    // velocity added through a nodal round trip (IMEXRK.h:96-107)
+    void adjoint_totals(const std::vector<C>* const below[4], const std::vector<C>* const above[4], T f) {
+        std::vector<C>* tot[4] = {&u1t, &u2t, &u3t, &bt};
+        for (int i = 0; i < 4; ++i)
+            for (size_t e = 0; e < nm; ++e) (*tot[i])[e] = (T(1) - f) * (*above[i])[e] + f * (*below[i])[e];
+        to_nodal(u1t, U1_tot);
+        per_nodal_stack([&](size_t o) {
+            for (int j = 0; j < N3; ++j) U1_tot[o + j] += Ubar[j];
+        });
+        to_modal(U1_tot, u1t);
+    }
+    // explicit terms alone, for tests: r = BuildRHSAdjoint with the nodal
+    // fields as they are; the state is left untouched
+    void adjoint_rhs(const std::vector<C>* const below[4], const std::vector<C>* const above[4], T f) {
+        adjoint_totals(below, above, f);
+        update_adjoint_variables();
+        zero(r1); zero(r2); zero(r3); zero(rB);
+        build_rhs_adjoint();
+    }
+    // r = BuildRHSLinear about the SetBackground state, for tests
+    void linear_rhs() {
+        zero(r1); zero(r2); zero(r3); zero(rB);
+        build_rhs_linear();
+    }
+
+    // PrepareRunAdjoint, IMEXRK.h:183-199 (without the image folders)
+    void prepare_adjoint() {
+        zero(p);
+        populate_nodal();
+    }
+
+    // TimeStepAdjoint, IMEXRK.h:86-126.  below/above: the direct state at the
+    // end and at the start of the (backward) step, u1, u2, u3, b.
+    void time_step_adjoint(const std::vector<C>* const below[4], const std::vector<C>* const above[4]) {
+        T interpFrac = 0;
+        for (int k = 0; k < 3; ++k) {
+            adjoint_totals(below, above, interpFrac);
+            update_adjoint_variables();
+            explicit_rk(k);
+            build_rhs_adjoint();
+            finish_rhs(k);
+            crank_nicolson(k);
+            remove_divergence(1.0 / double(h[k]));
+            filter_all();
+            populate_nodal();
+            interpFrac += h[k] / T(deltaT);
+        }
+    }
+
     // InnerProd (modal), Integration.h:117-146 with IntegrateVertically,
     // Integration.h:5-35.  Accumulates in T like the reference.
     T inner_prod(const std::vector<C>& a, const std::vector<C>& bb, int bc) const {
@@ -1117,6 +1370,40 @@ void sfo_step(void* p, int n) {
 }
 void sfo_step_linear(void* p, int n) {
     dispatch(static_cast<Handle*>(p), [&](auto& s) { for (int i = 0; i < n; ++i) s.time_step_linear(); return 0; });
+}
+void sfo_prepare_adjoint(void* p) {
+    dispatch(static_cast<Handle*>(p), [&](auto& s) { s.prepare_adjoint(); return 0; });
+}
+// One TimeStepAdjoint (mode 0), or only the explicit terms of the adjoint at
+// fraction frac (mode 1) or of the linearised equations (mode 2) into r1..rB.
+// below/above: 4 modal fields each (u1, u2, u3, b) in the reference layout;
+// a null field is zero.
+void sfo_adjoint(void* p, int mode, const double* const* below, const double* const* above, double frac) {
+    dispatch(static_cast<Handle*>(p), [&](auto& s) {
+        if (mode == 2) {
+            s.linear_rhs();
+            return 0;
+        }
+        using T = typename std::decay<decltype(s)>::type::C::value_type;
+        using C = std::complex<T>;
+        std::vector<C> bl[4], ab[4];
+        const std::vector<C>* pb[4];
+        const std::vector<C>* pa[4];
+        for (int i = 0; i < 4; ++i) {
+            bl[i].assign(s.nm, C(0));
+            ab[i].assign(s.nm, C(0));
+            for (size_t e = 0; e < s.nm; ++e) {
+                if (below[i]) bl[i][e] = C(T(below[i][2 * e]), T(below[i][2 * e + 1]));
+                if (above[i]) ab[i][e] = C(T(above[i][2 * e]), T(above[i][2 * e + 1]));
+            }
+            pb[i] = &bl[i];
+            pa[i] = &ab[i];
+        }
+        using T2 = typename std::decay<decltype(s)>::type::C::value_type;
+        if (mode == 1) s.adjoint_rhs(pb, pa, T2(frac));
+        else s.time_step_adjoint(pb, pa);
+        return 0;
+    });
 }
 void sfo_energy(void* p, double* ke, double* pe) {
     dispatch(static_cast<Handle*>(p), [&](auto& s) { *ke = s.ke(); *pe = s.pe(); return 0; });

@@ -55,7 +55,7 @@ EXPORTED = [
     "sf_field_download", "sf_field_copy", "sf_field_zero", "sf_field_scale", "sf_field_axpby",
     "sf_field_zero_ends", "sf_field_neumann_ends", "sf_field_filter", "sf_to_modal", "sf_to_nodal",
     "sf_ddx", "sf_ddy", "sf_ddz", "sf_inner_prod", "sf_state_field", "sf_step", "sf_step_linear",
-    "sf_set_linear_background", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
+    "sf_set_linear_background", "sf_step_adjoint", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
     "sf_energy", "sf_cfl", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
     "sf_batch_create", "sf_batch_destroy", "sf_batch_last_error", "sf_batch_step", "sf_batch_time_steps",
     "sf_batch_info",
@@ -107,6 +107,7 @@ def lib():
         "sf_step": [vp, i],
         "sf_step_linear": [vp, i],
         "sf_set_linear_background": [vp, vp, vp, vp, vp],
+        "sf_step_adjoint": [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)],
         "sf_stage": [vp, i, i],
         "sf_remove_divergence": [vp, d],
         "sf_filter_all": [vp],
@@ -338,6 +339,27 @@ class IMEXRK:
     def SetLinearBackground(self, u1, u2, u3, b):
         """IMEXRK::SetBackground(u1, u2, u3, b) with modal Field objects."""
         self._chk(lib().sf_set_linear_background(self.h, u1.h, u2.h if u2 is not None else None, u3.h, b.h))
+
+    def TimeStepAdjoint(self, below, above):
+        """IMEXRK::TimeStepAdjoint (IMEXRK.h:86-126).  below/above: sequences of
+        modal Field objects (u1, u2, u3, b) of this solver -- the direct state at
+        the end and at the start of the backward step; u2 may be None in 2D."""
+        def arr(fs):
+            a = (ctypes.c_void_p * 4)()
+            for i, f in enumerate(fs):
+                a[i] = f.h if f is not None else None
+            return a
+        self._chk(lib().sf_step_adjoint(self.h, arr(below), arr(above)))
+
+    def PrepareRunAdjoint(self):
+        """The set-up of StateVector::AdjointEvolve (StateVector.cpp:158-172) with
+        PrepareRunAdjoint (IMEXRK.h:183-199): FilterAll, PopulateNodalVariables,
+        RemoveDivergence(0), zero pressure, PopulateNodalVariables."""
+        self.FilterAll()
+        self.PopulateNodalVariables()
+        self.RemoveDivergence(0.0)
+        self.state("p").Zero()
+        self.PopulateNodalVariables()
 
     def RemoveDivergence(self, pressureMultiplier=1.0):
         self._chk(lib().sf_remove_divergence(self.h, float(pressureMultiplier)))

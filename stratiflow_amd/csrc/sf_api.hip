@@ -585,6 +585,109 @@ struct Ctx {
         to_nodal(mf(SF_B), static_cast<T*>(state[SF_NODAL_B].d));
     }
 
+    // ---- adjoint step (IMEXRK.h:86-126) ----
+    // nodal: direct totals, their horizontal derivatives, explicit terms;
+    // modal: transformed explicit terms and one derivative scratch field.
+    // Allocated on the first adjoint step.
+    T* adjT[4] = {nullptr};
+    T* adjDX[4] = {nullptr};
+    T* adjDY[4] = {nullptr};
+    T* adjQ[12] = {nullptr};
+    V* adjM[12] = {nullptr};
+    V* adjD = nullptr;
+
+    int adj_terms() const { return three ? 12 : 6; }
+    bool has_field(int i) const { return three || i != 1; }
+
+    void adjoint_alloc() {
+        if (adjD) return;
+        for (int i = 0; i < 4; ++i) {
+            if (!has_field(i)) continue;
+            adjT[i] = dalloc<T>(nnodal);
+            adjDX[i] = dalloc<T>(nnodal);
+            if (three) adjDY[i] = dalloc<T>(nnodal);
+        }
+        for (int q = 0; q < adj_terms(); ++q) {
+            adjQ[q] = dalloc<T>(nnodal);
+            adjM[q] = dalloc<V>(nmodal);
+        }
+        adjD = dalloc<V>(nmodal);
+    }
+
+    // below/above: the direct state (u1, u2, u3, b; device modal fields of this
+    // context, u2 null in 2D) at the end and at the start of the backward step
+    void step_adjoint(const V* const below[4], const V* const above[4]) {
+        adjoint_alloc();
+        V* tot[4] = {mf(SF_U1_TOT), mf(SF_U2_TOT), mf(SF_U3_TOT), mf(SF_B_TOT)};
+        T* nod[4] = {static_cast<T*>(state[SF_NODAL_U1].d), static_cast<T*>(state[SF_NODAL_U2].d),
+                     static_cast<T*>(state[SF_NODAL_U3].d), static_cast<T*>(state[SF_NODAL_B].d)};
+        const T* ubar = zc + size_t(ZC_UBAR) * N3p;
+        T frac = T(0);
+        for (int k = 0; k < 3; ++k) {
+            // the direct state at this stage, with the background velocity
+            // added through a nodal round trip (IMEXRK.h:96-107)
+            for (int i = 0; i < 4; ++i)
+                if (has_field(i)) check(launch_lerp<T>(tot[i], above[i], below[i], frac, nmodal, stream), "lerp");
+            to_nodal(tot[0], adjT[0]);
+            check(launch_add_profile<T>(adjT[0], ubar, nnodal, N3, stream), "add_profile");
+            to_modal(adjT[0], tot[0]);
+            // UpdateAdjointVariables (IMEXRK.h:461-500) and the nodal
+            // derivatives of BuildRHSAdjoint (IMEXRK.cpp:312-360)
+            for (int i = 0; i < 4; ++i) {
+                if (!has_field(i)) continue;
+                to_nodal(tot[i], adjT[i]);
+                check(launch_dmul<T>(adjD, tot[i], A1, A2, N3, kx, ky, 0, zc, N3p, stream), "ddx");
+                to_nodal(adjD, adjDX[i]);
+                if (three) {
+                    check(launch_dmul<T>(adjD, tot[i], A1, A2, N3, kx, ky, 1, zc, N3p, stream), "ddy");
+                    to_nodal(adjD, adjDY[i]);
+                }
+            }
+            AdjArgs<T> a{};
+            a.N3 = N3;
+            a.N3p = N3p;
+            a.A1 = A1;
+            a.n = nnodal;
+            for (int i = 0; i < 4; ++i) {
+                a.U[i] = has_field(i) ? nod[i] : nullptr;
+                a.Tt[i] = adjT[i];
+                a.DX[i] = adjDX[i];
+                a.DY[i] = adjDY[i];
+            }
+            a.u3t = tot[2];
+            a.bt = tot[3];
+            a.zc = zc;
+            a.Ri = T(Ri);
+            for (int q = 0; q < adj_terms(); ++q) a.Q[q] = adjQ[q];
+            check(launch_adj_products<T>(a, three, stream), "adj_products");
+            for (int q = 0; q < adj_terms(); ++q) to_modal(adjQ[q], adjM[q]);
+            AdjCombine<T> c{};
+            c.N3 = N3;
+            c.A1 = A1;
+            c.A2 = A2;
+            c.three = three ? 1 : 0;
+            c.kx = kx;
+            c.ky = ky;
+            for (int q = 0; q < adj_terms(); ++q) c.M[q] = adjM[q];
+            for (int i = 0; i < 4; ++i) c.out[i] = has_field(i) ? nl[i] : nullptr;
+            check(launch_adj_combine<T>(c, stream), "adj_combine");
+            // ExplicitRK, FinishRHS, CrankNicolson, RemoveDivergence(1/h), FilterAll
+            const bool store_r = k < 2;
+            ModalArgs<T> m = stage_modal_args(k, store_r, rcur);
+            m.skip_linear = 1;
+            check(launch_modal<T>(m, stream), "modal");
+            if (store_r) {
+                rcur = 1 - rcur;
+                bind_r();
+            }
+            // PopulateNodalVariables: the nodal fields follow the new state
+            for (int i = 0; i < 4; ++i)
+                if (has_field(i)) to_nodal(mf(SF_U1 + i), nod[i]);
+            use_snap = false;
+            frac += h[k] / T(dt);
+        }
+    }
+
     // dot products into scal[slot]
     void dot_async(const V* a, const V* b, int bc, int slot) {
         const T* w = zc + size_t(bc == SF_NEUMANN ? ZC_WN : ZC_WD) * N3p;
@@ -1298,6 +1401,32 @@ int sf_set_linear_background(sf_ctx* ctx, const sf_field* u1, const sf_field* u2
                 SF_REQUIRE(src[i]->kind == SF_MODAL && src[i]->n == d.n, SF_ERR_ARG, "background must be modal");
                 SF_HIP(hipMemcpyAsync(d.d, src[i]->d, d.n * sizeof(T), hipMemcpyDeviceToDevice, C.stream));
             }
+        });
+    });
+}
+
+int sf_step_adjoint(sf_ctx* ctx, const sf_field* const below[4], const sf_field* const above[4]) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(below);
+    SF_NULLCHK(above);
+    return guarded(ctx, [&]() {
+        with_ctx(ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            const cx<T>* b[4] = {nullptr};
+            const cx<T>* a[4] = {nullptr};
+            for (int i = 0; i < 4; ++i) {
+                if (!C.has_field(i)) continue;
+                for (int s = 0; s < 2; ++s) {
+                    const sf_field* f = s ? above[i] : below[i];
+                    SF_REQUIRE(f != nullptr, SF_ERR_ARG, "TimeStepAdjoint needs u1, u3, b (and u2 in 3D) below and above");
+                    SF_REQUIRE(f->ctx == ctx && f->kind == SF_MODAL && f->n == 2 * C.nmodal, SF_ERR_ARG,
+                               "adjoint direct states must be modal fields of this context");
+                    SF_REQUIRE(f->owned, SF_ERR_ARG,
+                               "adjoint direct states must be fields of their own, not the solver's state");
+                    (s ? a : b)[i] = static_cast<const cx<T>*>(f->d);
+                }
+            }
+            C.step_adjoint(b, a);
         });
     });
 }

@@ -16,7 +16,8 @@ struct ModalArgs {
     cx<T> *rn1, *rn2, *rn3, *rnB;        // explicit terms of this stage (null: not stored)
     const cx<T> *n1, *n2, *n3, *nB;      // nonlinear terms from the transform path
     T hz, mh, hb, cv, cb, Ri, pmult;
-    int mode;  // 0: full stage, 1: projection only (RemoveDivergence)
+    int mode;         // 0: full stage, 1: projection only (RemoveDivergence)
+    int skip_linear;  // 1: n1..nB are the complete explicit terms (BuildRHSAdjoint)
 };
 
 // Position of element (row, z, j1) of every spectral array: row is the active
@@ -101,6 +102,37 @@ hipError_t launch_ends(cx<T>* f, int A1, int A2, int N3, int bc_op, hipStream_t 
 template <typename T>
 hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, const T* kx, const T* ky, int axis,
                        const T* zc, int N3p, hipStream_t s);
+
+// adjoint step (sf_adjoint.hip)
+template <typename T>
+struct AdjArgs {
+    int N3, N3p, A1;
+    size_t n;            // nodal points N1*N2*N3
+    const T* U[4];       // adjoint nodal u1, u2, u3, b
+    const T* Tt[4];      // direct totals, nodal
+    const T* DX[4];      // nodal ddx of the totals
+    const T* DY[4];      // nodal ddy of the totals (3D)
+    const cx<T>* u3t;    // modal totals, for the horizontal averages
+    const cx<T>* bt;
+    const T* zc;
+    T Ri;
+    T* Q[12];            // nodal explicit terms
+};
+template <typename T>
+struct AdjCombine {
+    int N3, A1, A2, three;
+    const T *kx, *ky;
+    const cx<T>* M[12];  // modal transforms of the nodal explicit terms
+    cx<T>* out[4];       // explicit terms r1, r2, r3, rB
+};
+template <typename T>
+hipError_t launch_lerp(cx<T>* out, const cx<T>* above, const cx<T>* below, T f, size_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_add_profile(T* f, const T* prof, size_t n, int N3, hipStream_t s);
+template <typename T>
+hipError_t launch_adj_products(const AdjArgs<T>& a, bool three, hipStream_t s);
+template <typename T>
+hipError_t launch_adj_combine(const AdjCombine<T>& c, hipStream_t s);
 
 bool tz_supported(int n);
 // z levels per x-pass block for this grid (see sf_kernels.hip)

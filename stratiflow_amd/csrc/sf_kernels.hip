@@ -236,7 +236,8 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         issue(A.ro1, g0);  // first explicit term of the previous stage, in flight from here
         // BuildRHS linear terms, IMEXRK.cpp:140-147:
         //   r3 += Ri * ReinterpolateFull(b - <b>),  rB -= S * ReinterpolateDirichlet(u3)
-        {
+        // (BuildRHSAdjoint has none: its explicit terms arrive complete)
+        if (!A.skip_linear) {
             V bm[M];
             nb_prev<T, M>(bb, bm, lane);
             if (!zero_mode) {

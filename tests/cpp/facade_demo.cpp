@@ -49,6 +49,12 @@ int main(int argc, char** argv) {
     printf("pe20 %.9e\n", double(StateVector::solver().PE()));
     printf("traj_norm10 %.9e\n", double(traj[10].Norm()));
 
+    // adjoint of the first 10 steps, started from the initial state
+    StateVector adj;
+    x.AdjointEvolve(0.01f, 10, traj, adj);
+    printf("adj_ke %.9e\n", double(StateVector::solver().KE()));
+    printf("adj_pe %.9e\n", double(StateVector::solver().PE()));
+
     StateVector r;
     x.FullEvolve(0.5f, r, false, false);
     printf("full_norm %.9e\n", double(r.Norm()));

@@ -45,7 +45,26 @@ def test_facade_fixed_evolve_matches_oracle(tmp_path):
     o.populate_nodal()
     o.remove_divergence(0.0)
     o.set_dt(0.01)
-    o.step(20)
+    init = {n: o.get_modal(n) for n in ("u1", "u3", "b")}
+    states = []
+    for _ in range(20):
+        states.append({n: o.get_modal(n) for n in ("u1", "u3", "b")})
+        o.step(1)
     ke, pe = o.energy()
     assert abs(vals["ke20"] - ke) / ke < 1e-4
     assert abs(vals["pe20"] - pe) / pe < 1e-4
+    # AdjointEvolve (StateVector.cpp:158-187) over the first 10 steps
+    a = oracle.Oracle(N1, N2, N3, oracle.TWO_D, L1, 1.0, 10.0, 1000.0, 0.1, 1.0, precision=8)
+    a.set_background(ic.tanh_background(zf))
+    a.set_dt(0.01)
+    for n in init:
+        a.set_modal(n, init[n])
+    a.filter_all()
+    a.populate_nodal()
+    a.remove_divergence(0.0)
+    a.prepare_adjoint()
+    for step in range(10):
+        a.step_adjoint(states[9 - step], states[10 - step])
+    ke, pe = a.energy()
+    assert abs(vals["adj_ke"] - ke) / ke < 1e-4, (vals["adj_ke"], ke)
+    assert abs(vals["adj_pe"] - pe) / pe < 1e-4, (vals["adj_pe"], pe)
