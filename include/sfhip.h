@@ -136,6 +136,13 @@ int sf_ddy(const sf_field* in, sf_field* out);
 int sf_ddz(const sf_field* in, sf_field* out);
 /* InnerProd (Integration.h:117-146) accumulated in FP64 on the GPU */
 int sf_inner_prod(const sf_field* a, const sf_field* b, double* out);
+/* NodalField += Nodal1D (the per-stack broadcast of the expression templates,
+ * Field.h:1215-1253, as in U1_tot = U1 + U_, IMEXRK.cpp:150): profile holds
+ * N3 values */
+int sf_field_add_profile(sf_field* nodal, const double* profile);
+/* HorizontalAverage (Integration.h:37-42): the real part of the (0,0) stack
+ * of a modal field into N3 doubles */
+int sf_field_mean_profile(const sf_field* modal, double* out);
 
 /* ---- stepper (the IMEXRK solver state) ------------------------------- */
 /* borrow one of the solver's own fields (SF_U1 .. SF_B_TOT); the handle is
@@ -162,6 +169,11 @@ int sf_set_linear_background(sf_ctx* ctx, const sf_field* u1, const sf_field* u2
  * PopulateNodalVariables, RemoveDivergence(0), zero pressure,
  * PopulateNodalVariables (PrepareRunAdjoint, IMEXRK.h:183-199). */
 int sf_step_adjoint(sf_ctx* ctx, const sf_field* const below[4], const sf_field* const above[4]);
+/* IMEXRK::JoverK (IMEXRK.h:445-459): the mixing diagnostic J/K of
+ * UpdateAdjointVariables (IMEXRK.h:470-495), J = IntegrateVertically of the
+ * horizontal mean of -(B - <b>) W, K = 2, with B = ToNodal(b) (what every
+ * stage leaves in the reference's nodal B) and W = ToNodal(u3) */
+int sf_jover_k(sf_ctx* ctx, double* jover_k);
 /* one RK stage k of TimeStep (0..2) or TimeStepLinear (linear != 0) */
 int sf_stage(sf_ctx* ctx, int k, int linear);
 /* IMEXRK::RemoveDivergence (IMEXRK.cpp:43-73) */

@@ -261,6 +261,7 @@ private:
 };
 
 class NodalField;
+class Nodal1D;
 
 class ModalField : public DeviceField {  // Field.h:925-1207
 public:
@@ -324,6 +325,11 @@ public:
     void ToModal(ModalField& other) const {
         sfacade::check(sf_to_modal(dev(false), other.dev(true)), sfacade::ctx());
     }
+    // NodalField +/- Nodal1D, broadcast over every stack (Field.h:1215-1253)
+    NodalField& operator+=(const Nodal1D& profile);
+    NodalField& operator-=(const Nodal1D& profile);
+    using DeviceField::operator+=;
+    using DeviceField::operator-=;
     stratifloat Max() const {  // Field.h:771-786
         const double* p = host_read();
         double m = 0;
@@ -484,6 +490,50 @@ inline void Nodal1D::SetValue(std::function<stratifloat(stratifloat)> f, stratif
     }
 }
 
+inline NodalField& NodalField::operator+=(const Nodal1D& profile) {
+    std::vector<double> v(profile.Get().begin(), profile.Get().end());
+    sfacade::check(sf_field_add_profile(dev(true), v.data()), sfacade::ctx());
+    return *this;
+}
+inline NodalField& NodalField::operator-=(const Nodal1D& profile) {
+    std::vector<double> v(profile.Get().size());
+    for (size_t k = 0; k < v.size(); ++k) v[k] = -double(profile.Get()[k]);
+    sfacade::check(sf_field_add_profile(dev(true), v.data()), sfacade::ctx());
+    return *this;
+}
+
+// HorizontalAverage (Integration.h:37-42): the (0,0) stack, real part
+inline void HorizontalAverage(const ModalField& integrand, Nodal1D& into) {
+    std::vector<double> v(gridParams.N3);
+    sfacade::check(sf_field_mean_profile(integrand.dev(false), v.data()), sfacade::ctx());
+    for (int k = 0; k < gridParams.N3; ++k) into.Get()[k] = stratifloat(v[k]);
+}
+// RemoveHorizontalAverage (Integration.h:44-48)
+inline void RemoveHorizontalAverage(ModalField& field) {
+    for (int k = 0; k < gridParams.N3; ++k) field(0, 0, k) = complex(0);
+}
+// IntegrateVertically (Integration.h:5-35): Neumann profiles on rows 1..N3-2
+// with weights z[k+1]-z[k], Dirichlet profiles on rows 2..N3-2 with zf[k]-zf[k-1]
+inline stratifloat IntegrateVertically(const Nodal1D& U, stratifloat L3) {
+    (void)L3;  // the context's L3 is flowParams.L3
+    std::vector<double> z, zf;
+    VerticalPointsBoth(z, zf);
+    const int N = gridParams.N3;
+    stratifloat result = 0;
+    if (U.BC() == BoundaryCondition::Neumann) {
+        for (int k = 1; k < N - 1; k++) result += stratifloat(z[k + 1] - z[k]) * U.Get()[k];
+    } else {
+        for (int k = 2; k < N - 1; k++) result += stratifloat(zf[k] - zf[k - 1]) * U.Get()[k];
+    }
+    return result;
+}
+// IntegrateAllSpace (Integration.h:50-57)
+inline stratifloat IntegrateAllSpace(const ModalField& u, stratifloat L1, stratifloat L2, stratifloat L3) {
+    Nodal1D horzAve(u.BC());
+    HorizontalAverage(u, horzAve);
+    return IntegrateVertically(horzAve, L3) * L1 * L2;
+}
+
 // ddx / ddy / ddz (Stratiflow.h:51-80) evaluated into a new field
 inline NeumannModal ddx(const NeumannModal& f) {
     NeumannModal r;
@@ -567,6 +617,13 @@ public:
         U2.dev(true);
         U3.dev(true);
         B.dev(true);
+    }
+    // IMEXRK.h:445-459: mixing diagnostic from the nodal B and modal u3
+    stratifloat JoverK() {
+        flush_state();
+        double v = 0;
+        sfacade::check(sf_jover_k(c(), &v), c());
+        return stratifloat(v);
     }
     // IMEXRK.h:183-199 without the image folders
     void PrepareRunAdjoint() {
@@ -741,6 +798,58 @@ public:
     stratifloat Energy() const { return stratifloat(0.5 * DotF64(*this)); }
     stratifloat Norm() const { return stratifloat(std::sqrt(DotF64(*this))); }
 
+    // StateVector.h:151-157: <w, w> of the vorticity w = ddz(u1) - ddx(u3)
+    stratifloat Enstrophy() const {
+        DirichletModal w = ddz(u1);
+        w.axpby(-1, ddx(u3), 1);
+        return InnerProd(w, w, flowParams.L3);
+    }
+    // StateVector.h:159-169: InnerProd(u3, s) with s(1,0,j) = i, under u3's
+    // (Dirichlet) weights
+    stratifloat OtherAxis() const {
+        std::vector<double> z, zf;
+        VerticalPointsBoth(z, zf);
+        const DirichletModal& w = u3;
+        stratifloat result = 0;
+        for (int k = 2; k < gridParams.N3 - 1; k++) {
+            const stratifloat sum = 2 * std::real(w(1, 0, k) * std::conj(complex(0, 1)));
+            result += stratifloat(zf[k] - zf[k - 1]) * sum;
+        }
+        return result;
+    }
+    // StateVector.h:171-197: minimum over z of the gradient Richardson number
+    // of the horizontally averaged flow plus the background (which it sets to
+    // InitialU, as the reference does)
+    stratifloat MinimumRi() const {
+        solver().SetBackground(InitialU);
+        Neumann1D u_ave, b_ave;
+        HorizontalAverage(u1, u_ave);
+        HorizontalAverage(b, b_ave);
+        const int N = gridParams.N3;
+        for (int k = 0; k < N; k++) u_ave.Get()[k] = u_ave.Get()[k] + solver().U_.Get()[k];
+        // ddz of a Neumann profile (Differentiation.cpp:36-47): rows 0, 1 are zero
+        std::vector<double> z, zf;
+        VerticalPointsBoth(z, zf);
+        auto ddz1 = [&](const Neumann1D& f, int k) -> stratifloat {
+            if (k < 2) return 0;
+            const stratifloat inv = stratifloat(1.0 / (zf[k] - zf[k - 1]));
+            return inv * f.Get()[k] - inv * f.Get()[k - 1];
+        };
+        Neumann1D Ri_g;
+        for (int k = 0; k < N; k++) {
+            const stratifloat dbdz = ddz1(b_ave, k), dudz = ddz1(u_ave, k);
+            Ri_g.Get()[k] = flowParams.Ri * (dbdz + 1) / (dudz * dudz);
+        }
+        return Ri_g.Min();
+    }
+    // StateVector.h:423-466: add or remove U_ = tanh(z) and B_ = z through a
+    // nodal round trip
+    void AddBackground() { Background(1); }
+    void RemoveBackground() { Background(-1); }
+    // StateVector.h:476-500: the reference writes heat maps with matplotlib;
+    // here the directory is created and nothing is drawn
+    void PlotAll(std::string directory) const { MakeCleanDir(directory); }
+
     void Zero() {
         u1.Zero();
         if (gridParams.ThirdDimension()) u2.Zero();
@@ -902,7 +1011,117 @@ public:
         solver().UpdateForTimestep();
     }
 
+    // StateVector.h:275-413: load a 2D .fields file of grid K1 x K2 x K3 written
+    // with L3 = 10 and interpolate it linearly in z onto this grid (modes
+    // j1 < min(K1/2+1, N1/2+1); clamped beyond the old points; Neumann ends
+    // copied).  The file's ToModal runs in a temporary context of its grid.
+    template <int K1, int K2, int K3>
+    void LoadAndInterpolate(const std::string& filename) {
+        static_assert(K2 == 1, "LoadAndInterpolate is two-dimensional (StateVector.h:303-306)");
+        if (gridParams.N2 != 1 || gridParams.ThirdDimension())
+            throw std::runtime_error("LoadAndInterpolate needs a 2D grid (StateVector.h:303-306)");
+        Zero();
+        std::ifstream in(filename, std::ios::in | std::ios::binary);
+        if (!in) throw std::runtime_error("cannot open " + filename);
+        sf_grid g = {K1, 1, K3, SF_TWO_D};
+        sf_flow f = sfacade::flow_c();
+        f.L3 = 10;
+        sf_ctx* t = nullptr;
+        sfacade::check(sf_ctx_create(&g, &f, SF_FACADE_PRECISION, &t), nullptr);
+        std::unique_ptr<sf_ctx, int (*)(sf_ctx*)> hold(t, sf_ctx_destroy);
+        const int M1old = K1 / 2 + 1;
+        std::vector<double> loaded[4];
+        const int bcs[4] = {SF_NEUMANN, SF_NEUMANN, SF_DIRICHLET, SF_NEUMANN};
+        for (int i = 0; i < 4; ++i) {
+            const size_t count = size_t(K1) * size_t(K3);
+            std::vector<stratifloat> raw(count);
+            in.read(reinterpret_cast<char*>(raw.data()), sizeof(stratifloat) * raw.size());
+            std::vector<double> nod(raw.begin(), raw.end());
+            sf_field *fn = nullptr, *fm = nullptr;
+            sfacade::check(sf_field_alloc(t, SF_NODAL, bcs[i], &fn), t);
+            std::unique_ptr<sf_field, int (*)(sf_field*)> hn(fn, sf_field_free);
+            sfacade::check(sf_field_alloc(t, SF_MODAL, bcs[i], &fm), t);
+            std::unique_ptr<sf_field, int (*)(sf_field*)> hm(fm, sf_field_free);
+            sfacade::check(sf_field_upload(fn, nod.data()), t);
+            sfacade::check(sf_to_modal(fn, fm), t);
+            loaded[i].assign(size_t(2) * M1old * K3, 0.0);
+            sfacade::check(sf_field_download(fm, loaded[i].data()), t);
+        }
+        std::vector<double> zo, zfo, zn, zfn;
+        zo.assign(K3, 0.0);
+        zfo.assign(K3, 0.0);
+        sfacade::check(sf_vertical_points(t, zo.data(), zfo.data()), t);
+        VerticalPointsBoth(zn, zfn);
+        auto at = [&](int i, int j1, int k3) {
+            const double* q = &loaded[i][2 * (size_t(j1) * K3 + k3)];
+            return complex(stratifloat(q[0]), stratifloat(q[1]));
+        };
+        // bracketing old points and weights for a new point z (StateVector.h:316-345)
+        auto bracket = [&](const std::vector<double>& old, int last, stratifloat zz, int& kb, int& ka,
+                           stratifloat& wb, stratifloat& wa) {
+            ka = 0;
+            stratifloat zb, za;
+            do {
+                ka++;
+                kb = ka - 1;
+                zb = stratifloat(old[kb]);
+                za = stratifloat(old[ka]);
+            } while (za < zz && ka < last);
+            wa = (zz - zb) / (za - zb);
+            wb = (za - zz) / (za - zb);
+            if (wa > 1) {
+                wa = 1;
+                wb = 0;
+            }
+            if (wb > 1) {
+                wb = 1;
+                wa = 0;
+            }
+        };
+        const int N3 = gridParams.N3;
+        for (int j1 = 0; j1 < std::min(M1old, gridParams.N1 / 2 + 1); j1++) {
+            for (int j3 = 0; j3 < N3; j3++) {  // Neumann fields u1, u2, b
+                int kb, ka;
+                stratifloat wb, wa;
+                bracket(zfo, K3 - 2, stratifloat(zfn[j3]), kb, ka, wb, wa);
+                u1(j1, 0, j3) = wb * at(0, j1, kb) + wa * at(0, j1, ka);
+                u2(j1, 0, j3) = wb * at(1, j1, kb) + wa * at(1, j1, ka);
+                b(j1, 0, j3) = wb * at(3, j1, kb) + wa * at(3, j1, ka);
+            }
+            u1(j1, 0, 0) = complex(u1(j1, 0, 1));
+            u2(j1, 0, 0) = complex(u2(j1, 0, 1));
+            b(j1, 0, 0) = complex(b(j1, 0, 1));
+            u1(j1, 0, N3 - 1) = complex(u1(j1, 0, N3 - 2));
+            u2(j1, 0, N3 - 1) = complex(u2(j1, 0, N3 - 2));
+            b(j1, 0, N3 - 1) = complex(b(j1, 0, N3 - 2));
+            for (int j3 = 1; j3 < N3; j3++) {  // Dirichlet field u3
+                int kb, ka;
+                stratifloat wb, wa;
+                bracket(zo, K3 - 1, stratifloat(zn[j3]), kb, ka, wb, wa);
+                u3(j1, 0, j3) = wb * at(2, j1, kb) + wa * at(2, j1, ka);
+            }
+        }
+        EnforceBCs();
+    }
+
 private:
+    void Background(int sign) {
+        NeumannNodal U1, B;
+        Neumann1D U_, B_;
+        U_.SetValue(InitialU, flowParams.L3);
+        B_.SetValue([](stratifloat z) { return z; }, flowParams.L3);
+        u1.ToNodal(U1);
+        b.ToNodal(B);
+        if (sign > 0) {
+            U1 += U_;
+            B += B_;
+        } else {
+            U1 -= U_;
+            B -= B_;
+        }
+        U1.ToModal(u1);
+        B.ToModal(b);
+    }
     void CopyToSolver() const {  // StateVector.h:511-525
         solver().u1 = u1;
         if (gridParams.ThirdDimension()) solver().u2 = u2;

@@ -71,6 +71,8 @@ def lib():
         L.sfo_apply_op.argtypes = [vp, i, dp, dp]
         L.sfo_apply_op.restype = i
         L.sfo_prepare_adjoint.argtypes = [vp]
+        L.sfo_jover_k.argtypes = [vp]
+        L.sfo_jover_k.restype = d
         L.sfo_adjoint.argtypes = [vp, i, ctypes.POINTER(dp), ctypes.POINTER(dp), d]
         _lib = L
     return _lib
@@ -163,6 +165,10 @@ class Oracle:
 
     def step_linear(self, n=1):
         lib().sfo_step_linear(self.h, n)
+
+    def jover_k(self):
+        """IMEXRK::JoverK (IMEXRK.h:445-459) from the nodal B and the modal u3."""
+        return lib().sfo_jover_k(self.h)
 
     def prepare_adjoint(self):
         """PrepareRunAdjoint (IMEXRK.h:183-199): zero pressure, populate nodal fields."""

@@ -18,6 +18,7 @@
 //   * InnerProd / KE / PE                 Integration.h:5-35, 117-146; IMEXRK.h:395-411
 //   * TimeStepLinear / BuildRHSLinear     IMEXRK.cpp:26-41, 195-258; IMEXRK.h:334-352
 //   * TimeStepAdjoint / BuildRHSAdjoint   IMEXRK.h:86-126, 183-199, 461-500; IMEXRK.cpp:260-379
+//   * JoverK                              IMEXRK.h:445-459, 470-490
 //
 // Deliberate differences from the reference (all tolerance-level):
 //   * grid and operator coefficients are computed in double and rounded to T;
@@ -1149,6 +1150,27 @@ public:
         build_rhs_linear();
     }
 
+    // JoverK, IMEXRK.h:445-459, with the J integrand of UpdateAdjointVariables
+    // (IMEXRK.h:470-490): the nodal buoyancy B and the modal u3 of the state
+    T jover_k() {
+        std::vector<C> btot(nm);
+        std::vector<T> nd(nn);
+        to_modal(B, btot);      // nnTemp = B; ToModal(b_total)
+        to_nodal(btot, B_tot);  // b_total.ToNodal(B_tot)
+        to_nodal(u3, U3_tot);   // u3_total.ToNodal(U3_tot)
+        std::vector<T> bAve(N3);
+        for (int j = 0; j < N3; ++j) bAve[j] = btot[mi(0, 0, j)].real();
+        per_nodal_stack([&](size_t o) {
+            for (int j = 0; j < N3; ++j) nd[o + j] = (T(-1) * (B_tot[o + j] + T(-1) * bAve[j])) * U3_tot[o + j];
+        });
+        to_modal(nd, dirichletTemp);
+        // IntegrateVertically of the Dirichlet profile, Integration.h:22-34
+        T J = 0;
+        for (int k = 2; k < N3 - 1; ++k) J += T(g.zf[k] - g.zf[k - 1]) * dirichletTemp[mi(0, 0, k)].real();
+        const T K = 2;
+        return J / K;
+    }
+
     // PrepareRunAdjoint, IMEXRK.h:183-199 (without the image folders)
     void prepare_adjoint() {
         zero(p);
@@ -1404,6 +1426,9 @@ void sfo_adjoint(void* p, int mode, const double* const* below, const double* co
         else s.time_step_adjoint(pb, pa);
         return 0;
     });
+}
+double sfo_jover_k(void* p) {
+    return dispatch(static_cast<Handle*>(p), [&](auto& s) { return double(s.jover_k()); });
 }
 void sfo_energy(void* p, double* ke, double* pe) {
     dispatch(static_cast<Handle*>(p), [&](auto& s) { *ke = s.ke(); *pe = s.pe(); return 0; });

@@ -55,7 +55,8 @@ EXPORTED = [
     "sf_field_download", "sf_field_copy", "sf_field_zero", "sf_field_scale", "sf_field_axpby",
     "sf_field_zero_ends", "sf_field_neumann_ends", "sf_field_filter", "sf_to_modal", "sf_to_nodal",
     "sf_ddx", "sf_ddy", "sf_ddz", "sf_inner_prod", "sf_state_field", "sf_step", "sf_step_linear",
-    "sf_set_linear_background", "sf_step_adjoint", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
+    "sf_set_linear_background", "sf_step_adjoint", "sf_jover_k", "sf_field_add_profile", "sf_field_mean_profile",
+    "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
     "sf_energy", "sf_cfl", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
     "sf_batch_create", "sf_batch_destroy", "sf_batch_last_error", "sf_batch_step", "sf_batch_time_steps",
     "sf_batch_info",
@@ -108,6 +109,9 @@ def lib():
         "sf_step_linear": [vp, i],
         "sf_set_linear_background": [vp, vp, vp, vp, vp],
         "sf_step_adjoint": [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)],
+        "sf_jover_k": [vp, dp],
+        "sf_field_add_profile": [vp, dp],
+        "sf_field_mean_profile": [vp, dp],
         "sf_stage": [vp, i, i],
         "sf_remove_divergence": [vp, d],
         "sf_filter_all": [vp],
@@ -204,6 +208,17 @@ class Field:
     # Field / ModalField / NodalField operations
     def Zero(self):
         self.solver._chk(lib().sf_field_zero(self.h))
+
+    def add_profile(self, prof):
+        """NodalField += Nodal1D (N3 values broadcast over every stack)."""
+        a = np.ascontiguousarray(prof, dtype=np.float64)
+        self.solver._chk(lib().sf_field_add_profile(self.h, _p(a)))
+
+    def mean_profile(self):
+        """HorizontalAverage (Integration.h:37-42): Re of the (0,0) stack of a modal field."""
+        out = np.zeros(self.solver.N3)
+        self.solver._chk(lib().sf_field_mean_profile(self.h, _p(out)))
+        return out
 
     def assign(self, other):
         self.solver._chk(lib().sf_field_copy(self.h, other.h))
@@ -350,6 +365,12 @@ class IMEXRK:
                 a[i] = f.h if f is not None else None
             return a
         self._chk(lib().sf_step_adjoint(self.h, arr(below), arr(above)))
+
+    def JoverK(self):
+        """IMEXRK::JoverK (IMEXRK.h:445-459): mixing diagnostic of the current state."""
+        out = ctypes.c_double()
+        self._chk(lib().sf_jover_k(self.h, ctypes.byref(out)))
+        return out.value
 
     def PrepareRunAdjoint(self):
         """The set-up of StateVector::AdjointEvolve (StateVector.cpp:158-172) with

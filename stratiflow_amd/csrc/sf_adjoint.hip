@@ -214,11 +214,44 @@ hipError_t launch_adj_combine(const AdjCombine<T>& c, hipStream_t s) {
     return hipGetLastError();
 }
 
+// J integrand of UpdateAdjointVariables (IMEXRK.h:478-483): -(B - <b>) W with
+// <b> the real part of the (0,0) mode of the modal buoyancy
+template <typename T>
+__global__ __launch_bounds__(ABS) void k_mix_product(T* out, const T* Bt, const T* Wt, const cx<T>* bt, size_t n,
+                                                     int N3, int A1) {
+    for (size_t i = size_t(blockIdx.x) * ABS + threadIdx.x; i < n; i += size_t(gridDim.x) * ABS) {
+        const int z = int(i % size_t(N3));
+        const T bave = bt[size_t(z) * A1].x;
+        out[i] = (T(-1) * (Bt[i] + T(-1) * bave)) * Wt[i];
+    }
+}
+template <typename T>
+hipError_t launch_mix_product(T* out, const T* Bt, const T* Wt, const cx<T>* bt, size_t n, int N3, int A1,
+                              hipStream_t s) {
+    hipLaunchKernelGGL((k_mix_product<T>), dim3(grid_of<T>(n)), dim3(ABS), 0, s, out, Bt, Wt, bt, n, N3, A1);
+    return hipGetLastError();
+}
+
+// real part of the (0,0) stack of a modal field (HorizontalAverage,
+// Integration.h:37-42) into N3 doubles
+template <typename T>
+__global__ __launch_bounds__(ABS) void k_mean_profile(const cx<T>* f, double* out, int N3, int A1) {
+    const int z = blockIdx.x * ABS + threadIdx.x;
+    if (z < N3) out[z] = double(f[size_t(z) * A1].x);
+}
+template <typename T>
+hipError_t launch_mean_profile(const cx<T>* f, double* out, int N3, int A1, hipStream_t s) {
+    hipLaunchKernelGGL((k_mean_profile<T>), dim3((N3 + ABS - 1) / ABS), dim3(ABS), 0, s, f, out, N3, A1);
+    return hipGetLastError();
+}
+
 #define SF_ADJ_INST(T)                                                                                     \
     template hipError_t launch_lerp<T>(cx<T>*, const cx<T>*, const cx<T>*, T, size_t, hipStream_t);       \
     template hipError_t launch_add_profile<T>(T*, const T*, size_t, int, hipStream_t);                    \
     template hipError_t launch_adj_products<T>(const AdjArgs<T>&, bool, hipStream_t);                     \
-    template hipError_t launch_adj_combine<T>(const AdjCombine<T>&, hipStream_t);
+    template hipError_t launch_adj_combine<T>(const AdjCombine<T>&, hipStream_t);                         \
+    template hipError_t launch_mix_product<T>(T*, const T*, const T*, const cx<T>*, size_t, int, int, hipStream_t); \
+    template hipError_t launch_mean_profile<T>(const cx<T>*, double*, int, int, hipStream_t);
 
 SF_ADJ_INST(float)
 SF_ADJ_INST(double)

@@ -688,6 +688,56 @@ struct Ctx {
         }
     }
 
+    // ---- diagnostics ----
+    T* dprof = nullptr;  // one N3 profile on the device
+    T* jkN[3] = {nullptr};
+    V* jkM[2] = {nullptr};
+
+    // nodal += profile(z) (a NodalField plus a Nodal1D, Field.h:1215-1253)
+    void add_profile(T* nodal, const double* host) {
+        if (!dprof) dprof = dalloc<T>(N3p);
+        std::vector<T> h(N3);
+        for (int j = 0; j < N3; ++j) h[j] = T(host[j]);
+        SF_HIP(hipMemcpyAsync(dprof, h.data(), N3 * sizeof(T), hipMemcpyHostToDevice, stream));
+        check(launch_add_profile<T>(nodal, dprof, nnodal, N3, stream), "add_profile");
+        SF_HIP(hipStreamSynchronize(stream));
+    }
+
+    // HorizontalAverage (Integration.h:37-42): real part of the (0,0) stack
+    void mean_profile(const V* modal, double* out) {
+        check(launch_mean_profile<T>(modal, partial, N3, A1, stream), "mean_profile");
+        std::vector<double> h(N3);
+        SF_HIP(hipMemcpyAsync(h.data(), partial, N3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+        for (int j = 0; j < N3; ++j) out[j] = h[j];
+    }
+
+    // JoverK (IMEXRK.h:445-459) with the J integrand of UpdateAdjointVariables
+    // (IMEXRK.h:470-490): J = IntegrateVertically(<-(b - <b>) w>), K = 2.
+    // The reference reads the nodal B, which every stage leaves equal to
+    // ToNodal(b); the step here does not write nodal fields, so B is formed
+    // from the modal b of the state.
+    double jover_k() {
+        if (!jkM[0]) {
+            for (int i = 0; i < 3; ++i) jkN[i] = dalloc<T>(nnodal);
+            for (int i = 0; i < 2; ++i) jkM[i] = dalloc<V>(nmodal);
+        }
+        to_nodal(mf(SF_B), jkN[0]);  // B
+        to_modal(jkN[0], jkM[0]);    // b_total
+        to_nodal(jkM[0], jkN[0]);                                        // B_tot
+        to_nodal(mf(SF_U3), jkN[1]);                                     // U3_tot
+        check(launch_mix_product<T>(jkN[2], jkN[0], jkN[1], jkM[0], nnodal, N3, A1, stream), "mix_product");
+        to_modal(jkN[2], jkM[1]);
+        std::vector<double> Jint(N3);
+        mean_profile(jkM[1], Jint.data());
+        // IntegrateVertically of a Dirichlet profile (Integration.h:22-34), in T
+        const VGrid g = vertical_grid(L3, N3);
+        T J = 0;
+        for (int k = 2; k < N3 - 1; ++k) J += T(g.zf[k] - g.zf[k - 1]) * T(Jint[k]);
+        const T K = 2;
+        return double(J / K);
+    }
+
     // dot products into scal[slot]
     void dot_async(const V* a, const V* b, int bc, int slot) {
         const T* w = zc + size_t(bc == SF_NEUMANN ? ZC_WN : ZC_WD) * N3p;
@@ -1429,6 +1479,36 @@ int sf_step_adjoint(sf_ctx* ctx, const sf_field* const below[4], const sf_field*
             C.step_adjoint(b, a);
         });
     });
+}
+
+int sf_field_add_profile(sf_field* nodal, const double* profile) {
+    SF_NULLCHK(nodal);
+    SF_NULLCHK(profile);
+    return guarded(nodal->ctx, [&]() {
+        SF_REQUIRE(nodal->kind == SF_NODAL, SF_ERR_ARG, "a profile is added to a nodal field");
+        with_ctx(nodal->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            C.add_profile(static_cast<T*>(nodal->d), profile);
+        });
+    });
+}
+
+int sf_field_mean_profile(const sf_field* modal, double* out) {
+    SF_NULLCHK(modal);
+    SF_NULLCHK(out);
+    return guarded(modal->ctx, [&]() {
+        SF_REQUIRE(modal->kind == SF_MODAL, SF_ERR_ARG, "HorizontalAverage reads a modal field (Integration.h:38)");
+        with_ctx(modal->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            C.mean_profile(static_cast<const cx<T>*>(modal->d), out);
+        });
+    });
+}
+
+int sf_jover_k(sf_ctx* ctx, double* out) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(out);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { *out = C.jover_k(); }); });
 }
 
 int sf_stage(sf_ctx* ctx, int k, int linear) {

@@ -216,7 +216,7 @@ constexpr int fft_ldp() {
 }
 
 template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
-__device__ __forceinline__ void fft_pass1(int nl, LP lp) {
+__device__ __forceinline__ void fft_pass1(int nl, LP lp, const int tid) {
     using V = cx<T>;
     constexpr int R1 = FftPlan<N>::R1, R2 = FftPlan<N>::R2;
     constexpr int PER = (NLMAX * R2 + BS - 1) / BS;
@@ -224,7 +224,7 @@ __device__ __forceinline__ void fft_pass1(int nl, LP lp) {
     V v[PER][R1];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        const int g = int(threadIdx.x) + i * BS;
+        const int g = tid + i * BS;
         if (g < tot) {
             const int line = g / R2;
             const int j = g - line * R2;
@@ -237,7 +237,7 @@ __device__ __forceinline__ void fft_pass1(int nl, LP lp) {
     sync_lds();
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        const int g = int(threadIdx.x) + i * BS;
+        const int g = tid + i * BS;
         if (g < tot) {
             const int line = g / R2;
             const int j = g - line * R2;
@@ -249,9 +249,10 @@ __device__ __forceinline__ void fft_pass1(int nl, LP lp) {
     sync_lds();
 }
 
-// pass 2 twiddles: tw[j*R2 + r] = exp(-2 pi i r j / N), j < R1, r < R2
+// pass 2 twiddles: tw[j*R2 + r] = exp(-2 pi i r j / N), j < R1, r < R2 (each
+// lane reads its R2 - 1 twiddles from one contiguous run, in wide loads)
 template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
-__device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict__ tw, const int tid) {
     using V = cx<T>;
     constexpr int R1 = FftPlan<N>::R1, R2 = FftPlan<N>::R2;
     constexpr int PER = (NLMAX * R1 + BS - 1) / BS;
@@ -259,7 +260,7 @@ __device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict
     V v[PER][R2];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        const int g = int(threadIdx.x) + i * BS;
+        const int g = tid + i * BS;
         if (g < tot) {
             const int line = g / R1;
             const int j = g - line * R1;
@@ -279,7 +280,7 @@ __device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict
     sync_lds();
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        const int g = int(threadIdx.x) + i * BS;
+        const int g = tid + i * BS;
         if (g < tot) {
             const int line = g / R1;
             const int j = g - line * R1;
@@ -292,13 +293,13 @@ __device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict
 }
 
 template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
-__device__ __forceinline__ void fft_single(int nl, LP lp) {
+__device__ __forceinline__ void fft_single(int nl, LP lp, const int tid) {
     using V = cx<T>;
     constexpr int PER = (NLMAX + BS - 1) / BS;
     V v[PER][N];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        const int line = int(threadIdx.x) + i * BS;
+        const int line = tid + i * BS;
         if (line < nl) {
             const V* L = lp(line);
 #pragma unroll
@@ -309,7 +310,7 @@ __device__ __forceinline__ void fft_single(int nl, LP lp) {
     sync_lds();
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-        const int line = int(threadIdx.x) + i * BS;
+        const int line = tid + i * BS;
         if (line < nl) {
             V* L = lp(line);
 #pragma unroll
@@ -321,23 +322,26 @@ __device__ __forceinline__ void fft_single(int nl, LP lp) {
 
 // In-place FFT of nl <= NLMAX lines in LDS, each in natural order at lp(line)
 // with room for fft_ldp<N>() values.  The caller must have synchronised after
-// filling the lines; on return the results are visible to all threads.
+// filling the lines; on return the results are visible to all threads.  tid
+// is the thread's index in [0, BS) for the distribution of the work items (a
+// block may permute it to spread partially filled rounds over its waves).
 template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
-__device__ __forceinline__ void lds_fft_lines(int nl, LP lp, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void lds_fft_lines(int nl, LP lp, const cx<T>* __restrict__ tw,
+                                              const int tid = int(threadIdx.x)) {
     if constexpr (FftPlan<N>::two) {
-        fft_pass1<T, N, NLMAX, BS, SIGN>(nl, lp);
-        fft_pass2<T, N, NLMAX, BS, SIGN>(nl, lp, tw);
+        fft_pass1<T, N, NLMAX, BS, SIGN>(nl, lp, tid);
+        fft_pass2<T, N, NLMAX, BS, SIGN>(nl, lp, tw, tid);
     } else {
-        fft_single<T, N, NLMAX, BS, SIGN>(nl, lp);
+        fft_single<T, N, NLMAX, BS, SIGN>(nl, lp, tid);
         (void)tw;
     }
 }
 
 // NL contiguous lines at stride fft_ldp<N>()
 template <typename T, int N, int NL, int BS, int SIGN>
-__device__ __forceinline__ void lds_fft(cx<T>* buf, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void lds_fft(cx<T>* buf, const cx<T>* __restrict__ tw, const int tid = int(threadIdx.x)) {
     constexpr int LDP = fft_ldp<N>();
-    lds_fft_lines<T, N, NL, BS, SIGN>(NL, [buf](int line) { return buf + line * LDP; }, tw);
+    lds_fft_lines<T, N, NL, BS, SIGN>(NL, [buf](int line) { return buf + line * LDP; }, tw, tid);
 }
 
 // host: twiddle table of length N in the layout fft_pass2 reads (entry count N)

@@ -133,6 +133,11 @@ template <typename T>
 hipError_t launch_adj_products(const AdjArgs<T>& a, bool three, hipStream_t s);
 template <typename T>
 hipError_t launch_adj_combine(const AdjCombine<T>& c, hipStream_t s);
+template <typename T>
+hipError_t launch_mix_product(T* out, const T* Bt, const T* Wt, const cx<T>* bt, size_t n, int N3, int A1,
+                              hipStream_t s);
+template <typename T>
+hipError_t launch_mean_profile(const cx<T>* f, double* out, int N3, int A1, hipStream_t s);
 
 bool tz_supported(int n);
 // z levels per x-pass block for this grid (see sf_kernels.hip)

@@ -552,7 +552,11 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
     const int N3 = A.N3, A1 = A.A1;
     const int zbeg = bx * A.zch;
     const int zend = min(zbeg + A.zch, N3);
-    const int tid = threadIdx.x;
+    // Rounds that fill only part of the block (the products of 384 points,
+    // the FFT passes, the unpacking of A1 modes) fall to the first waves of
+    // the work-item order; rotating that order by whole waves from block to
+    // block spreads the extra rounds over the SIMDs of a CU.
+    const int tid = (int(threadIdx.x) + 64 * ((bx + n2) & (BS / 64 - 1))) & (BS - 1);
     auto slot = [&](int z) { return (z + 1 - zbeg) % 3; };
 
     // spectra of one level into registers: item = pair * A1 + j1
@@ -611,12 +615,14 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
         load(zbeg - 1 + l);
         store(l);
     }
+    // the spectra of level z+2 are loaded one whole level ahead: issued after
+    // the previous level's store, consumed by this level's store
+    if (zbeg + 1 < zend) load(zbeg + 2);
     sync_lds();
-    lds_fft_lines<T, N1, 3 * NI, BS, -1>(3 * NI, [ring](int line) { return ring + line * LDP; }, A.tw);
+    lds_fft_lines<T, N1, 3 * NI, BS, -1>(3 * NI, [ring](int line) { return ring + line * LDP; }, A.tw, tid);
 
     for (int z = zbeg; z < zend; ++z) {
         const bool more = z + 1 < zend;
-        if (more) load(z + 2);  // in flight while the products are formed
 
         // ---- all products of level z --------------------------------------
         {
@@ -711,7 +717,10 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
             }
         }
         sync_lds();
-        if (more) store(slot(z + 2));  // replaces level z-1
+        if (more) {
+            store(slot(z + 2));  // replaces level z-1
+            if (z + 2 < zend) load(z + 3);
+        }
         sync_lds();
         {
             const int s2 = slot(z + 2) * NI;
@@ -719,7 +728,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
                                              [prd, s2](int line) {
                                                  return prd + (line + (line >= NP ? s2 : 0)) * LDP;
                                              },
-                                             A.tw);
+                                             A.tw, tid);
         }
         // ---- separate the two real spectra of each product line, combine ---
         for (int k = tid; k < A1; k += BS) {
