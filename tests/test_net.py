@@ -108,8 +108,9 @@ def test_training_lowers_loss_and_identity_until_trained():
     net.apply(fin, outs)
     for f, o in zip(fin, outs):
         assert np.array_equal(f.download(), o.download())
-    # pairs of a fixed linear map applied per mode: q = 0.5 w (direction target 0)
-    rng = np.random.default_rng(2)
+    # pairs of a fixed per-field scaling q = (0.5 u1, 2 u3, b): the direction
+    # target q/RMS(q) - w/RMS(w) is a per-channel scaling of the input
+    scale = {"u1": 0.5, "u2": 1.0, "u3": 2.0, "b": 1.0}
     for k in range(6):
         w = [s.field(sf.MODAL, sf.DIRICHLET if n == "u3" else sf.NEUMANN) for n in ("u1", "u2", "u3", "b")]
         q = [s.field(sf.MODAL, sf.DIRICHLET if n == "u3" else sf.NEUMANN) for n in ("u1", "u2", "u3", "b")]
@@ -117,7 +118,7 @@ def test_training_lowers_loss_and_identity_until_trained():
         for i, n in enumerate(("u1", "u2", "u3", "b")):
             if n in pk:
                 w[i].upload(pk[n])
-                q[i].upload(0.5 * pk[n])
+                q[i].upload(scale[n] * pk[n])
         net.add_pair(w, q)
     assert net.samples() == 6 * 16
     l0, l1 = net.train(epochs=60, batch=32, lr=2e-3)
