@@ -143,11 +143,27 @@ int sf_field_add_profile(sf_field* nodal, const double* profile);
 /* HorizontalAverage (Integration.h:37-42): the real part of the (0,0) stack
  * of a modal field into N3 doubles */
 int sf_field_mean_profile(const sf_field* modal, double* out);
+/* InnerProd under the name of the FP64 dot product (same as sf_inner_prod) */
+int sf_dot_f64(const sf_field* a, const sf_field* b, double* out);
+/* the per-field part of StateVector::EnforceBCs (StateVector.h:415-421):
+ * ZeroEnds for a Dirichlet field, NeumannEnds for a Neumann field */
+int sf_field_enforce_bcs(sf_field* f);
+/* ModalField::PhaseShift (Field.h:1168-1174): stack(j1,j2) *= exp(i j1 shift) */
+int sf_phase_shift(sf_field* modal, double shift);
+/* RemoveAverage (Integration.h:72-75): stack(0,0) -= IntegrateAllSpace(f,1,1,L3)/L3/2 */
+int sf_remove_average(sf_field* modal);
 
 /* ---- stepper (the IMEXRK solver state) ------------------------------- */
 /* borrow one of the solver's own fields (SF_U1 .. SF_B_TOT); the handle is
  * owned by the context and must not be freed */
 int sf_state_field(sf_ctx* ctx, int which, sf_field** out);
+/* StateVector::CopyToSolver (StateVector.h:511-525): copy u1..p into the
+ * solver state (a NULL field zeroes its state field, as 2D zeroes u2) */
+int sf_state_bind(sf_ctx* ctx, const sf_field* u1, const sf_field* u2, const sf_field* u3, const sf_field* b,
+                  const sf_field* p);
+/* StateVector::CopyFromSolver (StateVector.h:532-543): copy the solver state
+ * into the non-NULL fields and enforce their boundary conditions */
+int sf_state_fetch(sf_ctx* ctx, sf_field* u1, sf_field* u2, sf_field* u3, sf_field* b, sf_field* p);
 /* IMEXRK::TimeStep (IMEXRK.cpp:8-24), nsteps times */
 int sf_step(sf_ctx* ctx, int nsteps);
 /* IMEXRK::TimeStepLinear (IMEXRK.cpp:26-41) about the state set with
@@ -184,6 +200,8 @@ int sf_filter_all(sf_ctx* ctx);
 int sf_populate_nodal(sf_ctx* ctx);
 /* IMEXRK::KE and PE (IMEXRK.h:395-411), FP64 reductions */
 int sf_energy(sf_ctx* ctx, double* ke, double* pe);
+/* KE and PE under the name of their FP64 reductions (same as sf_energy) */
+int sf_energy_f64(sf_ctx* ctx, double* ke, double* pe);
 /* IMEXRK::CFL (IMEXRK.h:355-374) without the dt update: returns the CFL
  * number for the current dt from the nodal fields (populates them) */
 int sf_cfl(sf_ctx* ctx, double* cfl);

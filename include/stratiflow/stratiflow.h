@@ -284,20 +284,18 @@ public:
     void NeumannEnds() { sfacade::check(sf_field_neumann_ends(dev(true)), sfacade::ctx()); }
     void Filter() { sfacade::check(sf_field_filter(dev(true)), sfacade::ctx()); }
 
-    // ModalField::PhaseShift (Field.h:1168-1174), on the host mirror
-    void PhaseShift(stratifloat shift) {
+    // ModalField::PhaseShift (Field.h:1168-1174)
+    void PhaseShift(stratifloat shift) { sfacade::check(sf_phase_shift(dev(true), shift), sfacade::ctx()); }
+    // ModalField::MakeMode2 (Field.h:1118-1127): zero the odd j1 stacks
+    void MakeMode2() {
         const GridParams& g = gridParams;
         double* p = host_write();
         for (int j2 = 0; j2 < g.N2; ++j2)
-            for (int j1 = 0; j1 < g.N1 / 3; ++j1) {
-                const std::complex<double> f = std::exp(std::complex<double>(0, j1 * double(shift)));
+            for (int j1 = 1; j1 < g.N1 / 2 + 1; j1 += 2)
                 for (int j3 = 0; j3 < g.N3; ++j3) {
                     double* q = p + 2 * index(j1, j2, j3);
-                    const std::complex<double> v = std::complex<double>(q[0], q[1]) * f;
-                    q[0] = v.real();
-                    q[1] = v.imag();
+                    q[0] = q[1] = 0;
                 }
-            }
     }
 
 protected:
@@ -526,6 +524,11 @@ inline stratifloat IntegrateVertically(const Nodal1D& U, stratifloat L3) {
         for (int k = 2; k < N - 1; k++) result += stratifloat(zf[k] - zf[k - 1]) * U.Get()[k];
     }
     return result;
+}
+// RemoveAverage (Integration.h:72-75)
+inline void RemoveAverage(ModalField& field, stratifloat L3) {
+    (void)L3;  // the context's L3 is flowParams.L3
+    sfacade::check(sf_remove_average(field.dev(true)), sfacade::ctx());
 }
 // IntegrateAllSpace (Integration.h:50-57)
 inline stratifloat IntegrateAllSpace(const ModalField& u, stratifloat L1, stratifloat L2, stratifloat L3) {
@@ -841,6 +844,13 @@ public:
             Ri_g.Get()[k] = flowParams.Ri * (dbdz + 1) / (dudz * dudz);
         }
         return Ri_g.Min();
+    }
+    // StateVector.h:468-474
+    void MakeMode2() {
+        u1.MakeMode2();
+        u2.MakeMode2();
+        u3.MakeMode2();
+        b.MakeMode2();
     }
     // StateVector.h:423-466: add or remove U_ = tanh(z) and B_ = z through a
     // nodal round trip

@@ -56,7 +56,8 @@ EXPORTED = [
     "sf_field_zero_ends", "sf_field_neumann_ends", "sf_field_filter", "sf_to_modal", "sf_to_nodal",
     "sf_ddx", "sf_ddy", "sf_ddz", "sf_inner_prod", "sf_state_field", "sf_step", "sf_step_linear",
     "sf_set_linear_background", "sf_step_adjoint", "sf_jover_k", "sf_field_add_profile", "sf_field_mean_profile",
-    "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
+    "sf_dot_f64", "sf_field_enforce_bcs", "sf_phase_shift", "sf_remove_average", "sf_state_bind", "sf_state_fetch",
+    "sf_energy_f64", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
     "sf_energy", "sf_cfl", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
     "sf_batch_create", "sf_batch_destroy", "sf_batch_last_error", "sf_batch_step", "sf_batch_time_steps",
     "sf_batch_info",
@@ -112,6 +113,13 @@ def lib():
         "sf_jover_k": [vp, dp],
         "sf_field_add_profile": [vp, dp],
         "sf_field_mean_profile": [vp, dp],
+        "sf_dot_f64": [vp, vp, dp],
+        "sf_field_enforce_bcs": [vp],
+        "sf_phase_shift": [vp, d],
+        "sf_remove_average": [vp],
+        "sf_state_bind": [vp, vp, vp, vp, vp, vp],
+        "sf_state_fetch": [vp, vp, vp, vp, vp, vp],
+        "sf_energy_f64": [vp, dp, dp],
         "sf_stage": [vp, i, i],
         "sf_remove_divergence": [vp, d],
         "sf_filter_all": [vp],
@@ -208,6 +216,18 @@ class Field:
     # Field / ModalField / NodalField operations
     def Zero(self):
         self.solver._chk(lib().sf_field_zero(self.h))
+
+    def PhaseShift(self, shift):
+        """ModalField::PhaseShift (Field.h:1168-1174)."""
+        self.solver._chk(lib().sf_phase_shift(self.h, float(shift)))
+
+    def RemoveAverage(self):
+        """RemoveAverage (Integration.h:72-75)."""
+        self.solver._chk(lib().sf_remove_average(self.h))
+
+    def EnforceBCs(self):
+        """ZeroEnds (Dirichlet) or NeumannEnds (Neumann), StateVector.h:415-421."""
+        self.solver._chk(lib().sf_field_enforce_bcs(self.h))
 
     def add_profile(self, prof):
         """NodalField += Nodal1D (N3 values broadcast over every stack)."""
@@ -365,6 +385,16 @@ class IMEXRK:
                 a[i] = f.h if f is not None else None
             return a
         self._chk(lib().sf_step_adjoint(self.h, arr(below), arr(above)))
+
+    def CopyToSolver(self, u1, u2, u3, b, p=None):
+        """StateVector::CopyToSolver (StateVector.h:511-525) with Field objects."""
+        h = lambda f: f.h if f is not None else None
+        self._chk(lib().sf_state_bind(self.h, h(u1), h(u2), h(u3), h(b), h(p)))
+
+    def CopyFromSolver(self, u1, u2, u3, b, p=None):
+        """StateVector::CopyFromSolver (StateVector.h:532-543) into Field objects."""
+        h = lambda f: f.h if f is not None else None
+        self._chk(lib().sf_state_fetch(self.h, h(u1), h(u2), h(u3), h(b), h(p)))
 
     def JoverK(self):
         """IMEXRK::JoverK (IMEXRK.h:445-459): mixing diagnostic of the current state."""

@@ -10,6 +10,8 @@
 // combines them with their horizontal derivatives into the explicit terms
 // (k_adj_combine), which the stage kernel then consumes with its linear
 // buoyancy terms switched off.  All of these kernels are HBM-bound streams.
+// The file also holds the small kernels of the diagnostics (JoverK, horizontal
+// averages) and of the field helpers (PhaseShift, RemoveAverage).
 #include "sf_kernels.h"
 
 namespace sf {
@@ -245,13 +247,51 @@ hipError_t launch_mean_profile(const cx<T>* f, double* out, int N3, int A1, hipS
     return hipGetLastError();
 }
 
+// ---- field helpers ----
+// stack(j1, j2) *= exp(i j1 shift) over the active modes (ModalField::PhaseShift,
+// Field.h:1168-1174); the active j1 equal their wavenumbers
+template <typename T>
+__global__ __launch_bounds__(ABS) void k_phase_shift(cx<T>* f, size_t n, int A1, double shift) {
+    for (size_t e = size_t(blockIdx.x) * ABS + threadIdx.x; e < n; e += size_t(gridDim.x) * ABS) {
+        const int j1 = int(e % size_t(A1));
+        double sn, cs;
+        sincos(double(j1) * shift, &sn, &cs);
+        const T c = T(cs), s = T(sn);
+        const cx<T> v = f[e];
+        cx<T> r;
+        r.x = v.x * c - v.y * s;
+        r.y = v.x * s + v.y * c;
+        f[e] = r;
+    }
+}
+template <typename T>
+hipError_t launch_phase_shift(cx<T>* f, size_t n, int A1, double shift, hipStream_t s) {
+    hipLaunchKernelGGL((k_phase_shift<T>), dim3(grid_of<T>(n)), dim3(ABS), 0, s, f, n, A1, shift);
+    return hipGetLastError();
+}
+
+// stack(0, 0) += value on every z (the real constant of RemoveAverage,
+// Integration.h:72-75)
+template <typename T>
+__global__ __launch_bounds__(ABS) void k_add_mean(cx<T>* f, int N3, int A1, T value) {
+    const int z = blockIdx.x * ABS + threadIdx.x;
+    if (z < N3) f[size_t(z) * A1].x += value;
+}
+template <typename T>
+hipError_t launch_add_mean(cx<T>* f, int N3, int A1, T value, hipStream_t s) {
+    hipLaunchKernelGGL((k_add_mean<T>), dim3((N3 + ABS - 1) / ABS), dim3(ABS), 0, s, f, N3, A1, value);
+    return hipGetLastError();
+}
+
 #define SF_ADJ_INST(T)                                                                                     \
     template hipError_t launch_lerp<T>(cx<T>*, const cx<T>*, const cx<T>*, T, size_t, hipStream_t);       \
     template hipError_t launch_add_profile<T>(T*, const T*, size_t, int, hipStream_t);                    \
     template hipError_t launch_adj_products<T>(const AdjArgs<T>&, bool, hipStream_t);                     \
     template hipError_t launch_adj_combine<T>(const AdjCombine<T>&, hipStream_t);                         \
     template hipError_t launch_mix_product<T>(T*, const T*, const T*, const cx<T>*, size_t, int, int, hipStream_t); \
-    template hipError_t launch_mean_profile<T>(const cx<T>*, double*, int, int, hipStream_t);
+    template hipError_t launch_mean_profile<T>(const cx<T>*, double*, int, int, hipStream_t);             \
+    template hipError_t launch_phase_shift<T>(cx<T>*, size_t, int, double, hipStream_t);                    \
+    template hipError_t launch_add_mean<T>(cx<T>*, int, int, T, hipStream_t);
 
 SF_ADJ_INST(float)
 SF_ADJ_INST(double)

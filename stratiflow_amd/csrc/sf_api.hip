@@ -1505,6 +1505,75 @@ int sf_field_mean_profile(const sf_field* modal, double* out) {
     });
 }
 
+int sf_dot_f64(const sf_field* a, const sf_field* b, double* out) { return sf_inner_prod(a, b, out); }
+
+int sf_field_enforce_bcs(sf_field* f) {
+    SF_NULLCHK(f);
+    return f->bc == SF_DIRICHLET ? sf_field_zero_ends(f) : sf_field_neumann_ends(f);
+}
+
+int sf_phase_shift(sf_field* f, double shift) {
+    SF_NULLCHK(f);
+    return guarded(f->ctx, [&]() {
+        SF_REQUIRE(f->kind == SF_MODAL, SF_ERR_ARG, "PhaseShift acts on modal fields (Field.h:1168)");
+        with_ctx(f->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            C.check(launch_phase_shift<T>(static_cast<cx<T>*>(f->d), C.nmodal, C.A1, shift, C.stream), "phase_shift");
+        });
+    });
+}
+
+int sf_remove_average(sf_field* f) {
+    SF_NULLCHK(f);
+    return guarded(f->ctx, [&]() {
+        SF_REQUIRE(f->kind == SF_MODAL, SF_ERR_ARG, "RemoveAverage acts on modal fields (Integration.h:72)");
+        with_ctx(f->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            std::vector<double> m(C.N3);
+            C.mean_profile(static_cast<const cx<T>*>(f->d), m.data());
+            // IntegrateAllSpace(field, 1, 1, L3) (Integration.h:5-35, 50-57), in T
+            const VGrid g = vertical_grid(C.L3, C.N3);
+            T integral = 0;
+            if (f->bc == SF_NEUMANN) {
+                for (int k = 1; k < C.N3 - 1; ++k) integral += T(g.z[k + 1] - g.z[k]) * T(m[k]);
+            } else {
+                for (int k = 2; k < C.N3 - 1; ++k) integral += T(g.zf[k] - g.zf[k - 1]) * T(m[k]);
+            }
+            const T avg = integral / T(C.L3) / T(2);
+            C.check(launch_add_mean<T>(static_cast<cx<T>*>(f->d), C.N3, C.A1, T(-avg), C.stream), "remove_average");
+        });
+    });
+}
+
+int sf_state_bind(sf_ctx* ctx, const sf_field* u1, const sf_field* u2, const sf_field* u3, const sf_field* b,
+                  const sf_field* p) {
+    SF_NULLCHK(ctx);
+    const sf_field* src[5] = {u1, u2, u3, b, p};
+    for (int i = 0; i < 5; ++i) {
+        sf_field* st = nullptr;
+        int rc = sf_state_field(ctx, SF_U1 + i, &st);
+        if (rc == SF_OK) rc = src[i] ? sf_field_copy(st, src[i]) : sf_field_zero(st);
+        if (rc != SF_OK) return rc;
+    }
+    return SF_OK;
+}
+
+int sf_state_fetch(sf_ctx* ctx, sf_field* u1, sf_field* u2, sf_field* u3, sf_field* b, sf_field* p) {
+    SF_NULLCHK(ctx);
+    sf_field* dst[5] = {u1, u2, u3, b, p};
+    for (int i = 0; i < 5; ++i) {
+        if (!dst[i]) continue;
+        sf_field* st = nullptr;
+        int rc = sf_state_field(ctx, SF_U1 + i, &st);
+        if (rc == SF_OK) rc = sf_field_copy(dst[i], st);
+        if (rc == SF_OK && i < 4) rc = sf_field_enforce_bcs(dst[i]);
+        if (rc != SF_OK) return rc;
+    }
+    return SF_OK;
+}
+
+int sf_energy_f64(sf_ctx* ctx, double* ke, double* pe) { return sf_energy(ctx, ke, pe); }
+
 int sf_jover_k(sf_ctx* ctx, double* out) {
     SF_NULLCHK(ctx);
     SF_NULLCHK(out);

@@ -138,6 +138,10 @@ hipError_t launch_mix_product(T* out, const T* Bt, const T* Wt, const cx<T>* bt,
                               hipStream_t s);
 template <typename T>
 hipError_t launch_mean_profile(const cx<T>* f, double* out, int N3, int A1, hipStream_t s);
+template <typename T>
+hipError_t launch_phase_shift(cx<T>* f, size_t n, int A1, double shift, hipStream_t s);
+template <typename T>
+hipError_t launch_add_mean(cx<T>* f, int N3, int A1, T value, hipStream_t s);
 
 bool tz_supported(int n);
 // z levels per x-pass block for this grid (see sf_kernels.hip)

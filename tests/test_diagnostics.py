@@ -144,3 +144,68 @@ def test_facade_diagnostics_and_load_and_interpolate(tmp_path):
     assert abs(vals["background_mid"] - np.tanh(zfn[N3 // 2])) < 1e-5
     assert vals["background_roundtrip"] < 1e-5
     assert os.path.isdir(st + "_plots")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,N1,N2,N3", [(O.TWO_D, 48, 1, 64), (O.THREE_D, 24, 12, 32)])
+def test_gpu_field_helpers(dim, N1, N2, N3):
+    """PhaseShift (Field.h:1168-1174), RemoveAverage (Integration.h:72-75),
+    EnforceBCs per field, CopyToSolver/CopyFromSolver (StateVector.h:511-543)."""
+    import stratiflow_amd as sf
+    from helpers import make_pair
+    s, o = make_pair(N1, N2, N3, dim, gpu_prec=8, amplitude=0.05, seed=11)
+    u1 = o.get_modal("u1")
+    A1 = N1 // 3
+    f = s.field(sf.MODAL, sf.NEUMANN)
+    f.upload(u1)
+    f.PhaseShift(0.7)
+    want = u1.copy()
+    want[:, :A1, :] *= np.exp(1j * 0.7 * np.arange(A1))[None, :, None]
+    assert np.abs(f.download() - want).max() < 1e-14
+
+    z, zf, _, _ = o.grid()
+    for name, bc in (("b", sf.NEUMANN), ("u3", sf.DIRICHLET)):
+        a = o.get_modal(name).copy()
+        a[0, 0, :] += 0.3  # a mean to remove
+        g = s.field(sf.MODAL, bc)
+        g.upload(a)
+        g.RemoveAverage()
+        m = a[0, 0, :].real
+        if bc == sf.NEUMANN:
+            integral = sum((z[k + 1] - z[k]) * m[k] for k in range(1, N3 - 1))
+        else:
+            integral = sum((zf[k] - zf[k - 1]) * m[k] for k in range(2, N3 - 1))
+        want = a.copy()
+        want[0, 0, :] -= integral / 10.0 / 2
+        assert np.abs(g.download() - want).max() < 1e-14, name
+
+    # EnforceBCs: Neumann ends copy rows 1 and N3-2, Dirichlet ends are zeroed
+    raw = np.random.default_rng(3).standard_normal(o.modal_shape) * (1 + 1j)
+    g = s.field(sf.MODAL, sf.NEUMANN)
+    g.upload(raw)
+    g.EnforceBCs()
+    r = g.download()
+    act = np.abs(r[..., 5]) > 0
+    assert np.array_equal(r[..., 0][act], r[..., 1][act]) and np.array_equal(r[..., -1][act], r[..., -2][act])
+    g = s.field(sf.MODAL, sf.DIRICHLET)
+    g.upload(raw)
+    g.EnforceBCs()
+    assert np.abs(g.download()[..., [0, 1, 2, N3 - 2, N3 - 1]]).max() == 0.0
+
+    # CopyToSolver / CopyFromSolver round trip
+    names = ["u1", "u2", "u3", "b", "p"]
+    src = {}
+    for n in names:
+        fl = s.field(sf.MODAL, sf.DIRICHLET if n == "u3" else sf.NEUMANN)
+        fl.upload(o.get_modal(n) if n != "p" else 0 * u1)
+        src[n] = fl
+    if dim == O.TWO_D:
+        src["u2"] = None
+    s.CopyToSolver(*[src[n] for n in names])
+    dst = {n: s.field(sf.MODAL, sf.DIRICHLET if n == "u3" else sf.NEUMANN) for n in names}
+    s.CopyFromSolver(*[dst[n] for n in names])
+    for n in ("u1", "u3", "b"):
+        t = s.field(sf.MODAL, sf.DIRICHLET if n == "u3" else sf.NEUMANN)
+        t.upload(o.get_modal(n))
+        t.EnforceBCs()  # CopyFromSolver ends with EnforceBCs
+        assert np.array_equal(dst[n].download(), t.download()), n
