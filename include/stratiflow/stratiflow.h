@@ -650,7 +650,20 @@ public:
     void SetBackground(const Neumann1D& velocity) {  // IMEXRK.h:321-324
         U_ = velocity;
         std::vector<double> u(U_.Get().begin(), U_.Get().end());
-        sfacade::check(sf_set_background(c(), u.data(), nullptr), c());
+        sfacade::check(sf_set_background(c(), u.data(), S_.empty() ? nullptr : S_.data()), c());
+    }
+    // Background stratification dB/dz at the Neumann points, which the
+    // reference fixes at 1 (IMEXRK.cpp:145-147); kept across SetBackground
+    // calls.  An empty function restores 1.
+    void SetBackgroundStratification(std::function<stratifloat(stratifloat)> dBdz) {
+        S_.clear();
+        if (dBdz) {
+            std::vector<double> z, zf;
+            VerticalPointsBoth(z, zf);
+            for (double v : zf) S_.push_back(double(dBdz(stratifloat(v))));
+        }
+        std::vector<double> u(U_.Get().begin(), U_.Get().end());
+        sfacade::check(sf_set_background(c(), u.data(), S_.empty() ? nullptr : S_.data()), c());
     }
     void SetBackground(std::function<stratifloat(stratifloat)> velocity) {  // IMEXRK.h:326-332
         Neumann1D Ubar;
@@ -725,6 +738,7 @@ public:
     NeumannModal u1, u2, b, p;  // IMEXRK.h:605-606
     DirichletModal u3;
     Neumann1D U_;                       // IMEXRK.h:609
+    std::vector<double> S_;             // dB/dz (empty: 1)
     NeumannNodal U1, U2, B;             // IMEXRK.h:618-619
     DirichletNodal U3;
 

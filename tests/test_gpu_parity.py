@@ -112,3 +112,33 @@ def test_steps_fp32_energy(shape):
         # stated bar: field L2 within 1e-4; guard: same order as the FP32 reference's own rounding
         assert eg < 1e-4, f
         assert eg <= max(10 * er, 1e-6), f
+
+
+def test_c1_energy_series_1000_steps():
+    """The 2D workload 128x1x128 (Re=1000, Ri=0.1, Pr=1, tanh U, dt=0.01, seeded
+    perturbation of amplitude 1e-3) for 1000 steps: the FP32 GPU KE/PE series
+    every 100 steps against the FP32 and FP64 oracles, to 1e-4 relative."""
+    import oracle
+    oracle.set_threads(16)
+    N1, N2, N3, dim = 128, 1, 128, 1
+    s, o32 = make_pair(N1, N2, N3, dim, gpu_prec=4, oracle_prec=4, amplitude=1e-3, seed=20261202)
+    _, o64 = make_pair(N1, N2, N3, dim, gpu_prec=4, oracle_prec=8, amplitude=1e-3, seed=20261202)
+    worst = 0.0
+    for chunk in range(10):
+        s.TimeStep(100)
+        o32.step(100)
+        o64.step(100)
+        ke, pe = s.energies()
+        k32, p32 = o32.energy()
+        k64, p64 = o64.energy()
+        e = max(abs(ke - k64) / k64, abs(pe - p64) / p64)
+        e32 = max(abs(k32 - k64) / k64, abs(p32 - p64) / p64)
+        _report("c1 step %d" % (100 * (chunk + 1)), ke=ke, pe=pe, gpu_vs64=e, ref32_vs64=e32)
+        assert abs(ke - k32) / k32 < 1e-4 and abs(pe - p32) / p32 < 1e-4
+        worst = max(worst, e)
+    assert worst < 1e-4
+    for f in fields(dim):
+        eg = rel_l2(s.get(f), o64.get_modal(f))
+        er = rel_l2(o32.get_modal(f), o64.get_modal(f))
+        _report("c1 fields", **{f + "_gpu": eg, f + "_ref32": er})
+        assert eg <= max(3 * er, 1e-6), f
