@@ -1,7 +1,7 @@
 // Newton-Krylov search for a steady state of the 2D stratified shear layer
 // (Newton-GMRES with FP64 Krylov algebra over FP32 or FP64 time stepping),
 // then the leading Arnoldi eigenvalues of the linearised map about it.
-//   newton_c3 <N1> <N3> <T> <plain steps> <amplitude> <K> <max Newton> <rel tol> <out dir>
+//   newton_c3 <N1> <N3> <T> <plain steps> <amplitude> <K> <max Newton> <rel tol> <out dir> [Delta]
 // Re=2000, Pr=1, Ri=0.12, U = tanh(z), B = tanh(z) (dB/dz = sech^2), dt = 0.01.
 // Prints:
 //   newton <step> <residual> <GMRES iterations> <seconds>
@@ -36,7 +36,10 @@ static StateVector Evolve(const StateVector& at, stratifloat dt, int nsteps) {
     sv.RemoveDivergence(0.0f);
     sv.deltaT = dt;
     sv.UpdateForTimestep();
-    for (int n = 0; n < nsteps; ++n) sv.TimeStep();
+    if (nsteps > 0) {
+        sv.TimeStep();  // synchronises dt and the state with the context
+        sfacade::check(sf_step(sfacade::ctx(), nsteps - 1), sfacade::ctx());
+    }
     StateVector r;
     r.u1 = sv.u1;
     r.u3 = sv.u3;
@@ -85,6 +88,7 @@ int main(int argc, char** argv) {
     const int maxNewton = std::atoi(argv[7]);
     const double reltol = std::atof(argv[8]);
     const std::string dir = argv[9];
+    const double Delta = argc > 10 ? std::atof(argv[10]) : 100.0;  // trust region (NewtonKrylov.h:29)
     gridParams = {N1, 1, N3, Dimensionality::TwoDimensional};
     flowParams.L1 = stratifloat(2 * 3.14159265358979 / 0.44);
     flowParams.L2 = 1;
@@ -112,6 +116,7 @@ int main(int argc, char** argv) {
     SteadyNewton nk;
     nk.verbose = false;
     nk.targetResidual = 0;  // the relative tolerance below decides
+    nk.Delta = Delta;
     nk.nsteps = int(std::lround(T / 0.01));
     nk.SetKrylovDimension(K);
     nk.gmresTolerance = 1e-3;
