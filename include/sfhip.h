@@ -265,6 +265,10 @@ int sf_net_get_params(sf_net* n, float* host);
 int sf_net_set_params(sf_net* n, const float* host);
 /* layers, padded channel counts C[0..layers] and profile length (N3) */
 int sf_net_geometry(const sf_net* n, int* layers, int* channels, int* z);
+/* dilation of each layer's five taps (layers values in [1, 8]; default 1): tap t
+ * of layer l reads position p + (t-2)*dilations[l], so dilations 1, 2, 4, 8
+ * widen the receptive field from 17 to 61 vertical points */
+int sf_net_set_dilations(sf_net* n, const int* dilations);
 /* raw tensors [profile][z][channel] (checks): forward of nprof profiles, and
  * the mean-squared-error loss against target with its parameter gradient */
 int sf_net_forward_raw(sf_net* n, const float* in, int nprof, float* out);

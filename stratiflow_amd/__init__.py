@@ -63,7 +63,7 @@ EXPORTED = [
     "sf_batch_info",
     "sf_net_create", "sf_net_destroy", "sf_net_last_error", "sf_net_apply", "sf_net_add_pair", "sf_net_clear_pairs",
     "sf_net_samples", "sf_net_train", "sf_net_param_count", "sf_net_get_params", "sf_net_set_params",
-    "sf_net_geometry", "sf_net_forward_raw", "sf_net_grad_raw",
+    "sf_net_geometry", "sf_net_forward_raw", "sf_net_grad_raw", "sf_net_set_dilations",
 ]
 KERNEL_CLASSES = ["yinv", "xpass", "yfwd", "modal"]
 
@@ -145,6 +145,7 @@ def lib():
         "sf_net_param_count": [vp, ctypes.POINTER(ctypes.c_size_t)],
         "sf_net_get_params": [vp, ctypes.POINTER(ctypes.c_float)],
         "sf_net_set_params": [vp, ctypes.POINTER(ctypes.c_float)],
+        "sf_net_set_dilations": [vp, ctypes.POINTER(ctypes.c_int)],
         "sf_net_geometry": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                             ctypes.POINTER(ctypes.c_int)],
         "sf_net_forward_raw": [vp, ctypes.POINTER(ctypes.c_float), i, ctypes.POINTER(ctypes.c_float)],
@@ -562,6 +563,11 @@ class Net:
         p = np.zeros(self.nparam, dtype=np.float32)
         self._chk(lib().sf_net_get_params(self.h, _fp(p)))
         return p
+
+    def set_dilations(self, dil):
+        """Dilation of each layer's taps (sf_net_set_dilations)."""
+        a = (ctypes.c_int * len(dil))(*[int(x) for x in dil])
+        self._chk(lib().sf_net_set_dilations(self.h, a))
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, dtype=np.float32)

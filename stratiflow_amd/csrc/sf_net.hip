@@ -36,6 +36,7 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int KT = 5;    // taps
+constexpr int NET_DMAX = 8;  // largest dilation of a layer
 constexpr int NB = 256;  // threads per block
 
 __device__ __forceinline__ bf16x8 ld8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
@@ -46,13 +47,14 @@ __device__ __forceinline__ bf16x8 zero8() {
     return z;
 }
 
-// out[p][co] = bias[co] + sum_{t,ci} W[t][co][ci] * in[p+t-2][ci], in taken as
-// zero outside the profile of p.  Optional: mask (out = 0 where mask <= 0),
-// FP32 copy, BF16 copy (with ReLU when relu != 0).  W is [KT][Cout][Cin].
+// out[p][co] = bias[co] + sum_{t,ci} W[t][co][ci] * in[p+(t-2)*dil][ci], in
+// taken as zero outside the profile of p.  Optional: mask (out = 0 where mask
+// <= 0), FP32 copy, BF16 copy (with ReLU when relu != 0).  W is [KT][Cout][Cin].
 template <int NT>
 __global__ __launch_bounds__(NB) void k_conv(const bf16* __restrict__ in, int Cin, const bf16* __restrict__ W,
                                              const float* __restrict__ bias, int P, int Z, float* __restrict__ out_f,
-                                             bf16* __restrict__ out_b, int relu, const float* __restrict__ mask) {
+                                             bf16* __restrict__ out_b, int relu, const float* __restrict__ mask,
+                                             int dil) {
     constexpr int Cout = NT * 16;
     const int lane = threadIdx.x & 63;
     const int p0 = (blockIdx.x * (NB / 64) + (threadIdx.x >> 6)) * 16;
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(NB) void k_conv(const bf16* __restrict__ in, int Ci
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int t = 0; t < KT; ++t) {
-        const int zz = z + t - 2;
+        const int zz = z + (t - 2) * dil;
         const bool ok = (p < P) && zz >= 0 && zz < Z;
         const bf16* arow = in + (size_t(b) * Z + (ok ? zz : 0)) * Cin + kg * 8;
         const bf16* wrow = W + size_t(t) * Cout * Cin + size_t(m) * Cin + kg * 8;
@@ -94,16 +96,16 @@ __global__ __launch_bounds__(NB) void k_conv(const bf16* __restrict__ in, int Ci
         }
 }
 
-// Partial weight gradients dW[t][co][ci] = sum_p g[p][co] * a[p+t-2][ci] (same
-// profile) over a grid-stride range of 64-position chunks, one partial per
-// block: part[block][KT][Cout][Cin].  Chunks go through LDS transposed, so that
-// the MFMA fragments read 8 consecutive positions.
+// Partial weight gradients dW[t][co][ci] = sum_p g[p][co] * a[p+(t-2)*dil][ci]
+// (same profile) over a grid-stride range of 64-position chunks, one partial
+// per block: part[block][KT][Cout][Cin].  Chunks go through LDS transposed, so
+// that the MFMA fragments read 8 consecutive positions.  dil <= NET_DMAX.
 template <int NTO, int NTI>
 __global__ __launch_bounds__(NB) void k_conv_dw(const bf16* __restrict__ g, const bf16* __restrict__ a, int P, int Z,
-                                                float* __restrict__ part) {
+                                                float* __restrict__ part, int dil) {
     constexpr int Cout = NTO * 16, Cin = NTI * 16;
-    constexpr int PC = 64;       // positions per chunk
-    constexpr int AW = PC + 8;   // staged a positions p0-2 .. p0+PC+1 (+pad)
+    constexpr int PC = 64;                    // positions per chunk
+    constexpr int AW = PC + 4 * NET_DMAX + 8;  // staged a positions p0-2*dil .. p0+PC-1+2*dil (+pad)
     constexpr int NTILE = KT * NTO * NTI;
     constexpr int TPW = (NTILE + 3) / 4;  // tiles per wave
     __shared__ bf16 gT[Cout * PC];
@@ -120,9 +122,9 @@ __global__ __launch_bounds__(NB) void k_conv_dw(const bf16* __restrict__ g, cons
             const int p = p0 + q;
             gT[co * PC + q] = p < P ? g[size_t(p) * Cout + co] : bf16(0.f);
         }
-        for (int idx = threadIdx.x; idx < (PC + 4) * Cin; idx += NB) {
+        for (int idx = threadIdx.x; idx < (PC + 4 * dil) * Cin; idx += NB) {
             const int q = idx / Cin, ci = idx - (idx / Cin) * Cin;
-            const int p = p0 - 2 + q;
+            const int p = p0 - 2 * dil + q;
             aT[ci * AW + q] = (p >= 0 && p < P) ? a[size_t(p) * Cin + ci] : bf16(0.f);
         }
         __syncthreads();
@@ -141,8 +143,8 @@ __global__ __launch_bounds__(NB) void k_conv_dw(const bf16* __restrict__ g, cons
                 for (int i = 0; i < 8; ++i) {
                     const int q = kb + kg * 8 + i;  // position p0+q
                     const int p = p0 + q;
-                    const int z = p - (p / Z) * Z + t - 2;
-                    bv[i] = (z >= 0 && z < Z) ? aT[(ji * 16 + m) * AW + q + t] : bf16(0.f);
+                    const int z = p - (p / Z) * Z + (t - 2) * dil;
+                    bv[i] = (z >= 0 && z < Z) ? aT[(ji * 16 + m) * AW + q + t * dil] : bf16(0.f);
                 }
                 acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[k], 0, 0, 0);
             }
@@ -382,6 +384,7 @@ struct Net {
     int nf = 0, L = 4, H = 64, Z = 0;
     int Creal_in = 0, Creal_out = 0;
     std::vector<int> C;  // padded channels: C[0] input .. C[L] output
+    std::vector<int> dil;  // dilation of each layer's taps (1: neighbours)
     std::vector<size_t> woff, boff;
     size_t nparam = 0;
     float *w = nullptr, *gr = nullptr, *am = nullptr, *av = nullptr;
@@ -454,6 +457,7 @@ struct Net {
         C.assign(L + 1, H);
         C[0] = 32;
         C[L] = 32;
+        dil.assign(L, 1);
         size_t off = 0;
         woff.resize(L);
         boff.resize(L);
@@ -520,14 +524,16 @@ struct Net {
     }
 
     void conv(int NT, const bf16* in, int Cin, const bf16* W, const float* bias, int P, float* out_f, bf16* out_b,
-              int relu, const float* mask) {
+              int relu, const float* mask, int d) {
         const dim3 grid((P + 63) / 64);
         switch (NT) {
             case 2:
-                hipLaunchKernelGGL(k_conv<2>, grid, dim3(NB), 0, st(), in, Cin, W, bias, P, Z, out_f, out_b, relu, mask);
+                hipLaunchKernelGGL(k_conv<2>, grid, dim3(NB), 0, st(), in, Cin, W, bias, P, Z, out_f, out_b, relu, mask,
+                                   d);
                 break;
             case 4:
-                hipLaunchKernelGGL(k_conv<4>, grid, dim3(NB), 0, st(), in, Cin, W, bias, P, Z, out_f, out_b, relu, mask);
+                hipLaunchKernelGGL(k_conv<4>, grid, dim3(NB), 0, st(), in, Cin, W, bias, P, Z, out_f, out_b, relu, mask,
+                                   d);
                 break;
             default:
                 throw Error(SF_ERR_ARG, "unsupported channel count");
@@ -540,7 +546,7 @@ struct Net {
         for (int l = 0; l < L; ++l) {
             const bool last = (l == L - 1);
             conv(C[l + 1] / 16, act[l], C[l], Wb[l], w + boff[l], P, pre[l], last ? nullptr : act[l + 1], last ? 0 : 1,
-                 nullptr);
+                 nullptr, dil[l]);
         }
     }
 
@@ -551,7 +557,7 @@ struct Net {
         const bf16* a = act[l];
 #define SF_DW(o, i)                                                                               \
     if (NTO == o && NTI == i) {                                                                   \
-        hipLaunchKernelGGL((k_conv_dw<o, i>), grid, dim3(NB), 0, st(), g, a, P, Z, part);         \
+        hipLaunchKernelGGL((k_conv_dw<o, i>), grid, dim3(NB), 0, st(), g, a, P, Z, part, dil[l]); \
     } else
         SF_DW(4, 2) SF_DW(4, 4) SF_DW(2, 4) SF_DW(2, 2) { throw Error(SF_ERR_ARG, "unsupported layer shape"); }
 #undef SF_DW
@@ -574,7 +580,7 @@ struct Net {
         for (int l = L - 1; l >= 0; --l) {
             dw(l, P);
             if (l > 0)  // gradient w.r.t. the ReLU input of layer l-1
-                conv(C[l] / 16, gbuf[l], C[l + 1], Wt[l], nullptr, P, nullptr, gbuf[l - 1], 0, pre[l - 1]);
+                conv(C[l] / 16, gbuf[l], C[l + 1], Wt[l], nullptr, P, nullptr, gbuf[l - 1], 0, pre[l - 1], dil[l]);
         }
         std::vector<double> h(gb);
         SF_HIP(hipMemcpyAsync(h.data(), dpart, gb * sizeof(double), hipMemcpyDeviceToHost, st()));
@@ -846,6 +852,16 @@ int sf_net_set_params(sf_net* n, const float* host) {
         SF_HIP(hipMemcpy(N.w, host, N.nparam * sizeof(float), hipMemcpyHostToDevice));
         N.cast_weights();
         SF_HIP(hipStreamSynchronize(N.st()));
+    });
+}
+
+int sf_net_set_dilations(sf_net* n, const int* dilations) {
+    if (!n || !dilations) return SF_ERR_ARG;
+    return net_call(n, [&](sf::net::Net& N) {
+        for (int l = 0; l < N.L; ++l)
+            SF_REQUIRE(dilations[l] >= 1 && dilations[l] <= sf::net::NET_DMAX, SF_ERR_ARG,
+                       "dilations must be in [1, 8]");
+        for (int l = 0; l < N.L; ++l) N.dil[l] = dilations[l];
     });
 }
 

@@ -29,21 +29,22 @@ def _rand_params(rng, channels, creal_in, creal_out, scale=1.0, last_zero=False)
     return convnet.pack(layers)
 
 
-def test_restatement_gradient_matches_finite_differences():
+@pytest.mark.parametrize("dil", [None, [1, 2, 3]])
+def test_restatement_gradient_matches_finite_differences(dil):
     from oracle import convnet
     rng = np.random.default_rng(0)
     ch = [4, 6, 5, 3]
     p = _rand_params(rng, ch, 4, 3)
-    x = rng.standard_normal((2, 7, 4))
-    tgt = rng.standard_normal((2, 7, 3))
-    loss, g = convnet.loss_and_grad(p, ch, x, tgt, 3, bf16=False)
+    x = rng.standard_normal((2, 11, 4))
+    tgt = rng.standard_normal((2, 11, 3))
+    loss, g = convnet.loss_and_grad(p, ch, x, tgt, 3, bf16=False, dil=dil)
     eps = 1e-6
     for k in rng.choice(p.size, 40, replace=False):
         pp, pm = p.copy(), p.copy()
         pp[k] += eps
         pm[k] -= eps
-        fd = (convnet.loss_and_grad(pp, ch, x, tgt, 3, bf16=False)[0]
-              - convnet.loss_and_grad(pm, ch, x, tgt, 3, bf16=False)[0]) / (2 * eps)
+        fd = (convnet.loss_and_grad(pp, ch, x, tgt, 3, bf16=False, dil=dil)[0]
+              - convnet.loss_and_grad(pm, ch, x, tgt, 3, bf16=False, dil=dil)[0]) / (2 * eps)
         assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, g[k])
 
 
@@ -58,6 +59,12 @@ def test_restatement_closed_forms():
     W[4] = np.eye(2)  # out[p] = in[p + 2]
     out = convnet.conv(x, W, None)
     assert np.allclose(out[0, :4], x[0, 2:]) and np.allclose(out[0, 4:], 0)
+    out = convnet.conv(x, W, None, 2)  # dilation 2: out[p] = in[p + 4]
+    assert np.allclose(out[0, :2], x[0, 4:]) and np.allclose(out[0, 2:], 0)
+    W = np.zeros((5, 2, 2))
+    W[0] = np.eye(2)  # out[p] = in[p - 2 d]
+    out = convnet.conv(x, W, None, 2)
+    assert np.allclose(out[0, 4:], x[0, :2]) and np.allclose(out[0, :4], 0)
     assert convnet.to_bf16(np.float32(1.0 + 2 ** -9)) == np.float32(1.0)
     assert convnet.to_bf16(np.float32(1.0 + 3 * 2 ** -9)) == np.float32(1.0 + 2 ** -7)
 
@@ -69,24 +76,27 @@ def _net(N1=48, N3=64, prec=4, seed=3):
 
 
 @pytest.mark.gpu
-def test_forward_and_gradient_match_restatement():
+@pytest.mark.parametrize("dil", [None, [1, 2, 4, 8]])
+def test_forward_and_gradient_match_restatement(dil):
     from oracle import convnet
     s, net = _net()
     assert net.channels == CH
+    if dil:
+        net.set_dilations(dil)
     rng = np.random.default_rng(1)
     p = _rand_params(rng, CH, 8, 6).astype(np.float32)
     net.set_params(p)
     x = np.zeros((5, net.Z, 32), np.float32)
     x[..., :8] = rng.standard_normal((5, net.Z, 8))
     out = net.forward_raw(x)
-    ref, _, _ = convnet.forward(p, CH, x, bf16=True)
+    ref, _, _ = convnet.forward(p, CH, x, bf16=True, dil=dil)
     err = np.abs(out - ref).max() / np.abs(ref).max()
     print("forward rel err", err)
     assert err < 2e-3
     tgt = np.zeros_like(out)
     tgt[..., :6] = rng.standard_normal((5, net.Z, 6))
     loss, g = net.grad_raw(x, tgt)
-    lref, gref = convnet.loss_and_grad(p, CH, x, tgt, 6, bf16=True)
+    lref, gref = convnet.loss_and_grad(p, CH, x, tgt, 6, bf16=True, dil=dil)
     print("loss gpu %.6e ref %.6e" % (loss, lref))
     assert abs(loss - lref) / lref < 1e-3
     eg = np.linalg.norm(g - gref) / np.linalg.norm(gref)
