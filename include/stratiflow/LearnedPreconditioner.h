@@ -16,6 +16,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "NewtonKrylov.h"
 #include "stratiflow.h"
@@ -40,6 +41,9 @@ public:
         out.EnforceBCs();
         return out;
     }
+    // dilation of each layer's taps (1 = neighbouring points); dilations 1, 2,
+    // 4, 8 let four layers see 61 vertical points instead of 17
+    void SetDilations(const std::vector<int>& d) { check(sf_net_set_dilations(net_, d.data())); }
     // a pair with A q = w
     void AddPair(const StateVector& w, const StateVector& q) {
         sf_field* a[4] = {w.u1.dev(false), w.u2.dev(false), w.u3.dev(false), w.b.dev(false)};
@@ -76,11 +80,17 @@ public:
     bool enabled = true;   // use the preconditioner once it is trained
     bool fallback = true;  // redo a solve with plain GMRES when the learned one misses the tolerance
     int fallbacks = 0;     // how many solves fell back
+    double lastLearnedResidual = 0;  // GMRES residual of the last preconditioned attempt
+    int lastLearnedIterations = 0;   // and its Krylov dimension
 
     // GMRES with the learned preconditioner, judged by the plain method: a
     // preconditioned solve that ends above the tolerance is repeated without it
     void SolveLinear(const StateVector& rhs, StateVector& x, double epsilon, double Delta_) override {
         Solver::SolveLinear(rhs, x, epsilon, Delta_);
+        if (enabled && pre_.Trained()) {
+            lastLearnedResidual = this->lastGmresResidual;
+            lastLearnedIterations = this->lastKrylovDimension;
+        }
         if (fallback && enabled && pre_.Trained() && !(this->lastGmresResidual < epsilon)) {
             ++fallbacks;
             const bool e = enabled;

@@ -1,12 +1,14 @@
 // Trains the learned preconditioner (include/stratiflow/LearnedPreconditioner.h)
 // on the operator applications of plain GMRES solves, then solves held-out
-// problems with and without it.  FP32 time stepping (the facade's default
+// problems with and without it (argv[7] > 0: dilated layers 1, 2, 4, 8).
+// FP32 time stepping (the facade's default
 // precision), FP64 Krylov algebra.  Prints, one per line:
 //   train <problem> <gmres iterations> <seconds>
 //   loss <first epoch> <last epoch> <samples> <train seconds>
 //   test <problem> <plain matvecs> <plain s> <learned matvecs> <learned s> <plain res> <learned res>
 //        <rel diff of the Newton updates> <fallbacks so far>
 //   hist <problem> <GMRES residual history of the plain solve>
+//   learned <problem> <Krylov dimension> <residual> of the preconditioned attempt
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -63,6 +65,7 @@ int main(int argc, char** argv) {
     StateVector::solver().SetBackground(InitialU);
 
     LearnedPreconditioner pre(64, 4, 1);
+    if (argc > 7 && std::atoi(argv[7]) > 0) pre.SetDilations({1, 2, 4, 8});
     Learned nk(pre);
     nk.verbose = false;
     nk.SetKrylovDimension(K);
@@ -107,6 +110,7 @@ int main(int argc, char** argv) {
         StateVector dp = xp - x0, dl = xl - xp;
         printf("test %d %ld %.4f %ld %.4f %.3e %.3e %.3e %d\n", i, mp, tp, ml, tl, rp, rl,
                sfkrylov::norm(dl) / sfkrylov::norm(dp), nk.fallbacks);
+        printf("learned %d %d %.3e\n", i, nk.lastLearnedIterations, nk.lastLearnedResidual);
     }
     return 0;
 }
