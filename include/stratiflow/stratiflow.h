@@ -1060,6 +1060,8 @@ public:
             const size_t count = size_t(K1) * size_t(K3);
             std::vector<stratifloat> raw(count);
             in.read(reinterpret_cast<char*>(raw.data()), sizeof(stratifloat) * raw.size());
+            if (!in) throw std::runtime_error(filename + " holds fewer than four " + std::to_string(K1) + "x1x" +
+                                              std::to_string(K3) + " fields");
             std::vector<double> nod(raw.begin(), raw.end());
             sf_field *fn = nullptr, *fm = nullptr;
             sfacade::check(sf_field_alloc(t, SF_NODAL, bcs[i], &fn), t);
