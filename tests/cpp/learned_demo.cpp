@@ -1,6 +1,7 @@
 // Trains the learned preconditioner (include/stratiflow/LearnedPreconditioner.h)
 // on the operator applications of plain GMRES solves, then solves held-out
-// problems with and without it (argv[7] > 0: dilated layers 1, 2, 4, 8).
+// problems with and without it (argv[7] > 0: dilated layers 1, 2, 4, 8;
+// argv[8]: rounds of retraining on the directions of preconditioned solves).
 // FP32 time stepping (the facade's default
 // precision), FP64 Krylov algebra.  Prints, one per line:
 //   train <problem> <gmres iterations> <seconds>
@@ -86,6 +87,21 @@ int main(int argc, char** argv) {
     const double t0 = now();
     pre.Train(epochs, 64, 1e-3, &l0, &l1);
     printf("loss %.6e %.6e %d %.3f\n", l0, l1, pre.Samples(), now() - t0);
+    // further rounds: harvest the directions of preconditioned solves of the
+    // training problems (the inputs the network meets in use), then retrain
+    const int rounds = argc > 8 ? std::atoi(argv[8]) : 0;
+    for (int r = 0; r < rounds; ++r) {
+        nk.harvest = true;
+        nk.enabled = true;
+        for (int i = 0; i < ntrain; ++i) {
+            StateVector x = problem(100 + i);
+            nk.Run(x);
+        }
+        nk.harvest = false;
+        const double t1 = now();
+        pre.Train(epochs, 64, 1e-3, &l0, &l1);
+        printf("loss %.6e %.6e %d %.3f\n", l0, l1, pre.Samples(), now() - t1);
+    }
 
     for (int i = 0; i < ntest; ++i) {
         const StateVector x0 = problem(1000 + i);
