@@ -8,6 +8,7 @@
 #include "stratiflow/stratiflow.h"
 
 int main(int argc, char** argv) {
+    setvbuf(stdout, nullptr, _IOLBF, 0);  // one line at a time, also into files
     if (argc < 4) return 2;
     gridParams = {48, 1, 96, Dimensionality::TwoDimensional};
     flowParams.L1 = stratifloat(2 * 3.14159265358979 / 0.44);

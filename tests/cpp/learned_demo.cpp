@@ -50,6 +50,7 @@ static StateVector problem(unsigned seed) {
 }
 
 int main(int argc, char** argv) {
+    setvbuf(stdout, nullptr, _IOLBF, 0);  // one line at a time, also into files
     const int ntrain = argc > 1 ? std::atoi(argv[1]) : 6;
     const int ntest = argc > 2 ? std::atoi(argv[2]) : 3;
     const int K = argc > 3 ? std::atoi(argv[3]) : 61;

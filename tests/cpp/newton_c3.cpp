@@ -79,6 +79,7 @@ protected:
 };
 
 int main(int argc, char** argv) {
+    setvbuf(stdout, nullptr, _IOLBF, 0);  // one line at a time, also into files
     if (argc < 10) return 2;
     const int N1 = std::atoi(argv[1]), N3 = std::atoi(argv[2]);
     const double T = std::atof(argv[3]);
