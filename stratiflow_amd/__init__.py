@@ -566,7 +566,9 @@ class Net:
 
     def set_dilations(self, dil):
         """Dilation of each layer's taps (sf_net_set_dilations)."""
-        a = (ctypes.c_int * len(dil))(*[int(x) for x in dil])
+        if len(dil) != self.layers:
+            raise ValueError("set_dilations needs one dilation per layer (%d)" % self.layers)
+        a =(ctypes.c_int * len(dil))(*[int(x) for x in dil])
         self._chk(lib().sf_net_set_dilations(self.h, a))
 
     def set_params(self, p):
