@@ -86,6 +86,9 @@ typedef struct sf_field sf_field;
 /* IMEXRK::IMEXRK (IMEXRK.h:37-81).  precision: 4 (float, the reference's
  * default stratifloat) or 8 (double, -DDOUBLE=On). */
 int sf_ctx_create(const sf_grid* grid, const sf_flow* flow, int precision, sf_ctx** out);
+/* Fields, networks and batches made from ctx must be freed first: they use
+ * its stream, and sf_field_free on a field of a destroyed context is a
+ * use after free. */
 int sf_ctx_destroy(sf_ctx* ctx);
 /* message of the last failed call on ctx (or of the last failed
  * sf_ctx_create when ctx is NULL) */

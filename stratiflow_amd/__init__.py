@@ -15,6 +15,7 @@ complex ``(N2, N1//2+1, N3)``, nodal fields real ``(N2, N1, N3)``.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -180,13 +181,18 @@ class Field:
         self.bc = bc
         self.owned = owned
 
-    def __del__(self):
+    def close(self):
+        """Free an owned field (sf_field_free); the solver does this for its
+        fields before it is closed."""
         if self.owned and getattr(self, "h", None):
-            try:
-                lib().sf_field_free(self.h)
-            except Exception:
-                pass
-            self.h = None
+            lib().sf_field_free(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     @property
     def shape(self):
@@ -292,9 +298,14 @@ class IMEXRK:
             raise SFError(rc, L.sf_last_error(None).decode())
         self.h = h
         self._state = {}
+        # owned fields, networks and batches made from this context: the C ABI
+        # requires them to be freed before the context is destroyed
+        self._deps = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "h", None):
+            for d in list(self._deps):
+                d.close()
             self._state = {}
             lib().sf_ctx_destroy(self.h)
             self.h = None
@@ -345,7 +356,9 @@ class IMEXRK:
     def field(self, kind, bc):
         h = ctypes.c_void_p()
         self._chk(lib().sf_field_alloc(self.h, kind, bc, ctypes.byref(h)))
-        return Field(self, h, kind, bc, True)
+        f = Field(self, h, kind, bc, True)
+        self._deps.add(f)
+        return f
 
     def state(self, name):
         if name not in self._state:
@@ -471,6 +484,8 @@ class Batch:
         if rc != 0:
             raise SFError(rc, L.sf_batch_last_error(None).decode())
         self.h = h
+        for r in self.runs:
+            r._deps.add(self)
 
     def _chk(self, rc):
         if rc != 0:
@@ -520,6 +535,7 @@ class Net:
         if rc != 0:
             raise SFError(rc, L.sf_net_last_error(None).decode())
         self.h = h
+        solver._deps.add(self)
         nl = ctypes.c_int()
         ch = (ctypes.c_int * 16)()
         z = ctypes.c_int()

@@ -55,3 +55,18 @@ def test_unsupported_grids(shape, code):
     with pytest.raises(sf.SFError) as e:
         sf.IMEXRK(N1, N2, N3, dim, 14.28, 7.14, 10.0, 1000.0, 0.1, 1.0)
     assert e.value.code == code
+
+
+def test_closing_a_solver_frees_its_fields_and_networks_first():
+    """sf_ctx_destroy requires the context's fields and networks to be freed
+    first; the Python solver does that for the objects it made, so dropping
+    them after close() is harmless."""
+    import gc
+    import stratiflow_amd as sf
+    s = sf.IMEXRK(48, 1, 64, sf.TWO_D, 14.28, 1.0, 10.0, 1000.0, 0.1, 1.0)
+    f = s.field(sf.MODAL, sf.NEUMANN)
+    net = sf.Net(s, hidden=32, layers=2, seed=0)
+    s.close()
+    assert f.h is None and net.h is None
+    del f, net
+    gc.collect()
