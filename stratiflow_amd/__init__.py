@@ -55,7 +55,7 @@ EXPORTED = [
     "sf_set_background", "sf_vertical_points", "sf_field_alloc", "sf_field_free", "sf_field_kind", "sf_field_upload",
     "sf_field_download", "sf_field_copy", "sf_field_zero", "sf_field_scale", "sf_field_axpby",
     "sf_field_zero_ends", "sf_field_neumann_ends", "sf_field_filter", "sf_to_modal", "sf_to_nodal",
-    "sf_ddx", "sf_ddy", "sf_ddz", "sf_inner_prod", "sf_state_field", "sf_step", "sf_step_linear",
+    "sf_ddx", "sf_ddy", "sf_ddz", "sf_field_diffusion_precondition", "sf_inner_prod", "sf_state_field", "sf_step", "sf_step_linear",
     "sf_set_linear_background", "sf_step_adjoint", "sf_jover_k", "sf_field_add_profile", "sf_field_mean_profile",
     "sf_dot_f64", "sf_field_enforce_bcs", "sf_phase_shift", "sf_remove_average", "sf_state_bind", "sf_state_fetch",
     "sf_energy_f64", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
@@ -105,6 +105,7 @@ def lib():
         "sf_ddx": [vp, vp],
         "sf_ddy": [vp, vp],
         "sf_ddz": [vp, vp],
+        "sf_field_diffusion_precondition": [vp, vp, d, d, d],
         "sf_inner_prod": [vp, vp, dp],
         "sf_state_field": [vp, i, vpp],
         "sf_step": [vp, i],
@@ -268,6 +269,11 @@ class Field:
 
     def Filter(self):
         self.solver._chk(lib().sf_field_filter(self.h))
+
+    def diffusion_precondition(self, out, tnu, delta, kmax=0.0):
+        """out = (tnu*(Dzz - k^2) - delta)^-1 ZeroEnds(self) - self/2 on modes with
+        |k| <= kmax, -self elsewhere (sf_field_diffusion_precondition)."""
+        self.solver._chk(lib().sf_field_diffusion_precondition(self.h, out.h, float(tnu), float(delta), float(kmax)))
 
     def ToModal(self, other):
         self.solver._chk(lib().sf_to_modal(self.h, other.h))

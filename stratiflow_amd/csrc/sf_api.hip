@@ -1398,6 +1398,35 @@ int sf_ddx(const sf_field* in, sf_field* out) { return deriv(in, out, 0); }
 int sf_ddy(const sf_field* in, sf_field* out) { return deriv(in, out, 1); }
 int sf_ddz(const sf_field* in, sf_field* out) { return deriv(in, out, 2); }
 
+int sf_field_diffusion_precondition(const sf_field* in, sf_field* out, double tnu, double delta, double kmax) {
+    SF_NULLCHK(in);
+    SF_NULLCHK(out);
+    return guarded(out->ctx, [&]() {
+        SF_REQUIRE(in->kind == SF_MODAL && same_shape(in, out) && in->bc == out->bc && in->d != out->d, SF_ERR_ARG,
+                   "the diffusion preconditioner maps a modal field to a distinct one of the same BC");
+        SF_REQUIRE(std::isfinite(tnu) && tnu >= 0 && std::isfinite(delta) && delta > 0 && std::isfinite(kmax),
+                   SF_ERR_ARG, "needs tnu >= 0, delta > 0 and a finite kmax");
+        with_ctx(out->ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            DiffPrecArgs<T> p{};
+            p.N3 = C.N3;
+            p.N3p = C.N3p;
+            p.A1 = C.A1;
+            p.A2 = C.A2;
+            p.dirichlet = in->bc == SF_DIRICHLET ? 1 : 0;
+            p.zc = C.zc;
+            p.kx2 = C.kx2;
+            p.ky2 = C.ky2;
+            p.in = static_cast<const cx<T>*>(in->d);
+            p.out = static_cast<cx<T>*>(out->d);
+            p.tnu = T(tnu);
+            p.delta = T(delta);
+            p.k2max = kmax < 0 ? T(-1) : T(kmax * kmax);
+            C.check(launch_diffprec<T>(p, C.stream), "diffusion preconditioner");
+        });
+    });
+}
+
 int sf_inner_prod(const sf_field* a, const sf_field* b, double* out) {
     SF_NULLCHK(a);
     SF_NULLCHK(b);

@@ -103,6 +103,19 @@ template <typename T>
 hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, const T* kx, const T* ky, int axis,
                        const T* zc, int N3p, hipStream_t s);
 
+// diffusion preconditioner of the Newton-Krylov solves (one wavefront per mode)
+template <typename T>
+struct DiffPrecArgs {
+    int N3, N3p, A1, A2, dirichlet;
+    const T* zc;
+    const T *kx2, *ky2;  // -kx^2, -ky^2 of the active modes
+    const cx<T>* in;
+    cx<T>* out;
+    T tnu, delta, k2max;
+};
+template <typename T>
+hipError_t launch_diffprec(const DiffPrecArgs<T>& p, hipStream_t s);
+
 // adjoint step (sf_adjoint.hip)
 template <typename T>
 struct AdjArgs {

@@ -1386,6 +1386,77 @@ hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, cons
     return hipGetLastError();
 }
 
+// Diffusion preconditioner, one wavefront per active mode (block = ja*A1 + j1).
+// Modes with kx^2 + ky^2 <= k2max:
+//   out = (tnu*(Dzz - k^2) - delta)^-1 ZeroEnds(in) - in/2
+// with the Neumannified or Dirichletified rows of the implicit solves
+// (IMEXRK.h:502-545), written as (I - (tnu/delta)*(Dzz - k^2)) x = -ZeroEnds(in)/delta
+// so that the Crank-Nicolson row builder and the column solver serve it;
+// other modes: out = -in.  (T L)^-1 - 1/2 is the Cayley approximation of
+// (exp(T L) - I)^-1 for the diffusion operator L = nu (Dzz - k^2).
+template <typename T, int M>
+__global__ __launch_bounds__(64) void k_diffprec(DiffPrecArgs<T> P) {
+    using V = cx<T>;
+    const int lane = threadIdx.x;
+    const int ja = blockIdx.x / P.A1;
+    const int j1 = blockIdx.x - ja * P.A1;
+    const int N = P.N3;
+    int row[M];
+    V x[M];
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+        row[t] = lane * M + t;
+        x[t] = row[t] < N ? P.in[sidx(ja, row[t], j1, N, P.A1)] : czero<V>();
+    }
+    const T kk = P.kx2[j1], kl = P.ky2[ja];
+    V y[M];
+    if (-(kk + kl) <= P.k2max) {  // wave-uniform
+        const bool dir = P.dirichlet != 0;
+        const T rs = T(-1) / P.delta;
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            const int j = row[t];
+            const bool end = j >= N || j <= 1 || j >= N - 2 || (dir && j == 2);
+            y[t] = end ? czero<V>() : cscale(rs, x[t]);
+        }
+        ModalArgs<T> A{};
+        A.N3 = N;
+        A.N3p = P.N3p;
+        A.zc = P.zc;
+        T a[M], b[M], c[M];
+        build_rows<T, M>(dir ? 1 : 0, A, row, kk, kl, false, P.tnu / P.delta, a, b, c);
+        TriFactor<T, M> F;
+        tri_factor<T, M>(F, a, b, c, lane);
+        tri_apply<T, M>(F, y, lane);
+#pragma unroll
+        for (int t = 0; t < M; ++t) y[t] = csub(y[t], cscale(T(0.5), x[t]));
+    } else {
+#pragma unroll
+        for (int t = 0; t < M; ++t) y[t] = cscale(T(-1), x[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < M; ++t)
+        if (row[t] < N) P.out[sidx(ja, row[t], j1, N, P.A1)] = y[t];
+}
+
+template <typename T>
+hipError_t launch_diffprec(const DiffPrecArgs<T>& p, hipStream_t s) {
+    const int M = (p.N3 + 63) / 64;
+    const unsigned nb = unsigned(p.A1) * unsigned(p.A2);
+    if (nb == 0 || p.in == p.out || !(p.delta > T(0))) return hipErrorInvalidValue;
+    switch (M) {
+#define SF_M(m)                                                                   \
+    case m:                                                                       \
+        hipLaunchKernelGGL((k_diffprec<T, m>), dim3(nb), dim3(64), 0, s, p);      \
+        break;
+        SF_M(1) SF_M(2) SF_M(3) SF_M(4) SF_M(5) SF_M(6) SF_M(7) SF_M(8)
+#undef SF_M
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 #define SF_INST(T)                                                                                                  \
     template hipError_t launch_modal<T>(const ModalArgs<T>&, hipStream_t);                                           \
     template hipError_t launch_modal_batch<T>(const ModalArgs<T>*, const int*, int, int, int, int, hipStream_t);      \
@@ -1401,7 +1472,8 @@ hipError_t launch_dmul(cx<T>* out, const cx<T>* in, int A1, int A2, int N3, cons
     template hipError_t launch_maxabs<T>(const T*, size_t, int, const T*, double*, double*, int, hipStream_t);       \
     template hipError_t launch_ends<T>(cx<T>*, int, int, int, int, hipStream_t);                                          \
     template hipError_t launch_dmul<T>(cx<T>*, const cx<T>*, int, int, int, const T*, const T*, int, const T*, int, \
-                                       hipStream_t);
+                                       hipStream_t);                                                                 \
+    template hipError_t launch_diffprec<T>(const DiffPrecArgs<T>&, hipStream_t);
 
 SF_INST(float)
 SF_INST(double)

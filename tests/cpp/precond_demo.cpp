@@ -1,0 +1,98 @@
+// Newton-GMRES solves of F(x) = Phi_T(x) - x with and without the diffusion
+// preconditioner (include/stratiflow/DiffusionPreconditioner.h), on the
+// held-out problems of learned_demo.cpp (64x1x64, T = 100 steps of 0.02).
+// FP32 time stepping, FP64 Krylov algebra.
+//   argv: ntest K tol Re delta kmax
+// Prints, one per line:
+//   test <problem> <plain matvecs> <plain s> <precond matvecs> <precond s> <plain res> <precond res>
+//        <|dx_precond - dx_plain| / |dx_plain|>
+//   total <plain matvecs> <precond matvecs> <plain converged> <precond converged>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "stratiflow/DiffusionPreconditioner.h"
+#include "stratiflow/NewtonKrylov.h"
+
+class FixedNewton : public NewtonKrylov<StateVector> {
+public:
+    stratifloat dt = stratifloat(0.02);
+    int nsteps = 100;
+
+protected:
+    StateVector EvalFunction(const StateVector& at) override {
+        StateVector s = at;
+        std::vector<StateVector> traj;
+        s.FixedEvolve(dt, nsteps + 1, traj);
+        StateVector r = traj[nsteps];
+        r -= at;
+        return r;
+    }
+};
+
+static double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static StateVector problem(unsigned seed) {
+    SeedRandom(seed);
+    StateVector x;
+    x.ExciteLowWavenumbers(stratifloat(1e-3 * (1.0 + 0.5 * (seed % 3))));
+    return x;
+}
+
+int main(int argc, char** argv) {
+    setvbuf(stdout, nullptr, _IOLBF, 0);
+    const int ntest = argc > 1 ? std::atoi(argv[1]) : 4;
+    const int K = argc > 2 ? std::atoi(argv[2]) : 61;
+    const double tol = argc > 3 ? std::atof(argv[3]) : 2e-2;
+    gridParams = {64, 1, 64, Dimensionality::TwoDimensional};
+    flowParams.L1 = stratifloat(2 * 3.14159265358979 / 0.44);
+    flowParams.L2 = 1;
+    flowParams.L3 = 10;
+    flowParams.Re = argc > 4 ? std::atof(argv[4]) : 100;
+    flowParams.Ri = stratifloat(0.1);
+    flowParams.Pr = 1;
+    const double delta = argc > 5 ? std::atof(argv[5]) : 1e-3;
+    const double kmax = argc > 6 ? std::atof(argv[6]) : 0.0;
+    StateVector::ResetForParams();
+    StateVector::solver().SetBackground(InitialU);
+
+    typedef WithDiffusionPreconditioner<FixedNewton> Pre;
+    Pre nk(DiffusionPreconditioner(0.02 * 100, delta, kmax));
+    nk.verbose = false;
+    nk.SetKrylovDimension(K);
+    nk.gmresTolerance = tol;
+    nk.maxNewtonSteps = 1;
+
+    long tp_mv = 0, tl_mv = 0;
+    int cp = 0, cl = 0;
+    for (int i = 0; i < ntest; ++i) {
+        const StateVector x0 = problem(1000 + i);
+        StateVector xp = x0, xl = x0;
+        nk.enabled = false;
+        long m0 = nk.matVecs;
+        double t = now();
+        nk.Run(xp);
+        const double tp = now() - t;
+        const long mp = nk.matVecs - m0;
+        const double rp = nk.lastGmresResidual;
+        nk.enabled = true;
+        m0 = nk.matVecs;
+        t = now();
+        nk.Run(xl);
+        const double tl = now() - t;
+        const long ml = nk.matVecs - m0;
+        const double rl = nk.lastGmresResidual;
+        StateVector dp = xp - x0, dl = xl - xp;
+        printf("test %d %ld %.4f %ld %.4f %.3e %.3e %.3e\n", i, mp, tp, ml, tl, rp, rl,
+               sfkrylov::norm(dl) / sfkrylov::norm(dp));
+        tp_mv += mp;
+        tl_mv += ml;
+        cp += rp < tol;
+        cl += rl < tol;
+    }
+    printf("total %ld %ld %d %d\n", tp_mv, tl_mv, cp, cl);
+    return 0;
+}
