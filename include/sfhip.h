@@ -140,14 +140,17 @@ int sf_ddz(const sf_field* in, sf_field* out);
 /* Diffusion preconditioner for Newton-Krylov solves of F(x) = Phi_T(x) - x
  * (no counterpart in the reference, whose GMRES is unpreconditioned:
  * NewtonKrylov.h:138-235).  For each active mode with kx^2 + ky^2 <= kmax^2:
- *   out = (tnu*(Dzz - k^2) - delta)^-1 ZeroEnds(in) - in/2,
+ *   out = (tnu*(Dzz - k^2) - i*tadv*kx*U(z) - delta)^-1 ZeroEnds(in) - in/2,
  * Dzz Neumannified or Dirichletified by the field's BC as in the implicit
  * solves (IMEXRK.h:502-545); tnu = T*nu (nu = 1/Re, or 1/(Re*Pr) for b),
- * delta > 0 regularises the conserved Neumann mean.  Other modes (all of
- * them when kmax < 0): out = -in.  (T L)^-1 - 1/2 approximates
- * (exp(T L) - I)^-1 for L = nu*(Dzz - k^2).  in and out: distinct modal
- * fields of one context and BC. */
-int sf_field_diffusion_precondition(const sf_field* in, sf_field* out, double tnu, double delta, double kmax);
+ * tadv = T to include the advection by the background U_(z) (0: diffusion
+ * only), delta > 0 regularises the conserved Neumann mean.  Other modes (all
+ * of them when kmax < 0): out = -in.  (T L)^-1 - 1/2 approximates
+ * (exp(T L) - I)^-1 for L = nu*(Dzz - k^2) - i*kx*U.  in and out: distinct
+ * modal fields of one context and BC.  With tadv != 0: N3 <= 512 (FP32) or
+ * 256 (FP64). */
+int sf_field_diffusion_precondition(const sf_field* in, sf_field* out, double tnu, double delta, double kmax,
+                                    double tadv);
 /* InnerProd (Integration.h:117-146) accumulated in FP64 on the GPU */
 int sf_inner_prod(const sf_field* a, const sf_field* b, double* out);
 /* NodalField += Nodal1D (the per-stack broadcast of the expression templates,

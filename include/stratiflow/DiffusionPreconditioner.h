@@ -2,15 +2,19 @@
 // Newton-GMRES driver on F(x) = Phi_T(x) - x (no counterpart in the reference,
 // whose GMRES is unpreconditioned: NewtonKrylov.h:138-235).
 //
-// The slow directions of A = Phi_T' - I are modes that mostly diffuse: for
-// them A ~ exp(T L) - I with L = nu (Dzz - k^2), whose eigenvalues approach 0
-// for the horizontally uniform profiles.  Apply(v) is, per field with its own
-// boundary condition and diffusivity (1/Re for the velocity, 1/(Re Pr) for b),
+// The slow directions of A = Phi_T' - I are modes that the background barely
+// advects and that mostly diffuse (near the critical level U = 0): for them
+// A ~ exp(T L) - I with L = nu (Dzz - k^2) - i kx U(z), eigenvalues near 0.
+// Apply(v) is, per field with its own boundary condition and diffusivity
+// (1/Re for the velocity, 1/(Re Pr) for b),
 //   (T L - delta)^-1 v - v/2   on modes with |k| <= kmax,
 //   -v                         on the others,
 // where (T L)^-1 - 1/2 is the Cayley approximation of (exp(T L) - I)^-1 and
-// delta > 0 keeps the conserved Neumann means finite.  One library call per
-// field (sf_field_diffusion_precondition), one wavefront per mode.
+// delta > 0 keeps the conserved Neumann means finite.  Without `advect` L is
+// the diffusion alone.  One library call per field
+// (sf_field_diffusion_precondition).  DiffusionPreconditioner(T, 0.3, 1e9,
+// true) is the measured setting for the 64x1x64 Newton problems of
+// tests/cpp/precond_demo.cpp.
 #pragma once
 
 #include <stdexcept>
@@ -21,9 +25,9 @@
 
 class DiffusionPreconditioner {
 public:
-    // T: the integration time of Phi_T
-    explicit DiffusionPreconditioner(double T, double delta = 1e-3, double kmax = 0)
-        : T_(T), delta_(delta), kmax_(kmax) {}
+    // T: the integration time of Phi_T; advect: include -i kx U(z) in L
+    explicit DiffusionPreconditioner(double T, double delta = 1e-3, double kmax = 0, bool advect = false)
+        : T_(T), delta_(delta), kmax_(kmax), advect_(advect) {}
 
     StateVector Apply(const StateVector& v) const {
         StateVector out;
@@ -43,11 +47,12 @@ public:
 
 private:
     void run(const sf_field* in, sf_field* out, double tnu) const {
-        if (sf_field_diffusion_precondition(in, out, tnu, delta_, kmax_) != SF_OK)
+        if (sf_field_diffusion_precondition(in, out, tnu, delta_, kmax_, advect_ ? T_ : 0.0) != SF_OK)
             throw std::runtime_error(std::string("sf_field_diffusion_precondition: ") +
                                      sf_last_error(sfacade::ctx()));
     }
     double T_, delta_, kmax_;
+    bool advect_;
 };
 
 // Adds the diffusion preconditioner to any NewtonKrylov<StateVector> driver:

@@ -105,7 +105,7 @@ def lib():
         "sf_ddx": [vp, vp],
         "sf_ddy": [vp, vp],
         "sf_ddz": [vp, vp],
-        "sf_field_diffusion_precondition": [vp, vp, d, d, d],
+        "sf_field_diffusion_precondition": [vp, vp, d, d, d, d],
         "sf_inner_prod": [vp, vp, dp],
         "sf_state_field": [vp, i, vpp],
         "sf_step": [vp, i],
@@ -270,10 +270,11 @@ class Field:
     def Filter(self):
         self.solver._chk(lib().sf_field_filter(self.h))
 
-    def diffusion_precondition(self, out, tnu, delta, kmax=0.0):
-        """out = (tnu*(Dzz - k^2) - delta)^-1 ZeroEnds(self) - self/2 on modes with
-        |k| <= kmax, -self elsewhere (sf_field_diffusion_precondition)."""
-        self.solver._chk(lib().sf_field_diffusion_precondition(self.h, out.h, float(tnu), float(delta), float(kmax)))
+    def diffusion_precondition(self, out, tnu, delta, kmax=0.0, tadv=0.0):
+        """out = (tnu*(Dzz - k^2) - i*tadv*kx*U(z) - delta)^-1 ZeroEnds(self) - self/2
+        on modes with |k| <= kmax, -self elsewhere (sf_field_diffusion_precondition)."""
+        self.solver._chk(lib().sf_field_diffusion_precondition(self.h, out.h, float(tnu), float(delta), float(kmax),
+                                                               float(tadv)))
 
     def ToModal(self, other):
         self.solver._chk(lib().sf_to_modal(self.h, other.h))

@@ -1398,14 +1398,16 @@ int sf_ddx(const sf_field* in, sf_field* out) { return deriv(in, out, 0); }
 int sf_ddy(const sf_field* in, sf_field* out) { return deriv(in, out, 1); }
 int sf_ddz(const sf_field* in, sf_field* out) { return deriv(in, out, 2); }
 
-int sf_field_diffusion_precondition(const sf_field* in, sf_field* out, double tnu, double delta, double kmax) {
+int sf_field_diffusion_precondition(const sf_field* in, sf_field* out, double tnu, double delta, double kmax,
+                                    double tadv) {
     SF_NULLCHK(in);
     SF_NULLCHK(out);
     return guarded(out->ctx, [&]() {
         SF_REQUIRE(in->kind == SF_MODAL && same_shape(in, out) && in->bc == out->bc && in->d != out->d, SF_ERR_ARG,
                    "the diffusion preconditioner maps a modal field to a distinct one of the same BC");
-        SF_REQUIRE(std::isfinite(tnu) && tnu >= 0 && std::isfinite(delta) && delta > 0 && std::isfinite(kmax),
-                   SF_ERR_ARG, "needs tnu >= 0, delta > 0 and a finite kmax");
+        SF_REQUIRE(std::isfinite(tnu) && tnu >= 0 && std::isfinite(delta) && delta > 0 && std::isfinite(kmax) &&
+                       std::isfinite(tadv),
+                   SF_ERR_ARG, "needs tnu >= 0, delta > 0, a finite kmax and a finite tadv");
         with_ctx(out->ctx, [&](auto& C) {
             using T = typename std::remove_reference<decltype(C.h[0])>::type;
             DiffPrecArgs<T> p{};
@@ -1415,6 +1417,7 @@ int sf_field_diffusion_precondition(const sf_field* in, sf_field* out, double tn
             p.A2 = C.A2;
             p.dirichlet = in->bc == SF_DIRICHLET ? 1 : 0;
             p.zc = C.zc;
+            p.kx = C.kx;
             p.kx2 = C.kx2;
             p.ky2 = C.ky2;
             p.in = static_cast<const cx<T>*>(in->d);
@@ -1422,6 +1425,9 @@ int sf_field_diffusion_precondition(const sf_field* in, sf_field* out, double tn
             p.tnu = T(tnu);
             p.delta = T(delta);
             p.k2max = kmax < 0 ? T(-1) : T(kmax * kmax);
+            p.tadv = T(tadv);
+            SF_REQUIRE(tadv == 0 || size_t(C.N3) * 16 * sizeof(cx<T>) <= 64 * 1024, SF_ERR_UNSUPPORTED,
+                       "the advected preconditioner supports N3 <= 512 (FP32) or 256 (FP64)");
             C.check(launch_diffprec<T>(p, C.stream), "diffusion preconditioner");
         });
     });

@@ -102,7 +102,7 @@ __host__ __device__ __forceinline__ V czero() {
     return r;
 }
 
-// Per-z coefficient table rows (see sf_context.cpp for their definition).
+// Per-z coefficient table rows (filled by the context set-up in sf_api.hip).
 enum ZCoef {
     ZC_DZZN_LO = 0,
     ZC_DZZN_DI,

@@ -108,10 +108,11 @@ template <typename T>
 struct DiffPrecArgs {
     int N3, N3p, A1, A2, dirichlet;
     const T* zc;
-    const T *kx2, *ky2;  // -kx^2, -ky^2 of the active modes
+    const T *kx, *kx2, *ky2;  // kx, -kx^2, -ky^2 of the active modes
     const cx<T>* in;
     cx<T>* out;
     T tnu, delta, k2max;
+    T tadv;  // T of the advection term -i kx U(z) (0: diffusion only)
 };
 template <typename T>
 hipError_t launch_diffprec(const DiffPrecArgs<T>& p, hipStream_t s);

@@ -1439,11 +1439,98 @@ __global__ __launch_bounds__(64) void k_diffprec(DiffPrecArgs<T> P) {
         if (row[t] < N) P.out[sidx(ja, row[t], j1, N, P.A1)] = y[t];
 }
 
+// The same preconditioner with the advection of each mode by the background,
+// L = nu (Dzz - k^2) - i kx U(z): complex coefficients, so one thread per mode
+// runs a Thomas sweep (the system is strictly diagonally dominant for
+// delta > 0) with its modified super-diagonal in LDS and the modified
+// right-hand side in out; ADV_T modes per block.
+constexpr int ADV_T = 16;
+template <typename T>
+__global__ __launch_bounds__(ADV_T) void k_advprec(DiffPrecArgs<T> P) {
+    using V = cx<T>;
+    extern __shared__ unsigned char adv_lds[];
+    V* cp = reinterpret_cast<V*>(adv_lds);  // [z][ADV_T]
+    const int t = threadIdx.x;
+    const int m = blockIdx.x * ADV_T + t;
+    if (m >= P.A1 * P.A2) return;  // no barriers below
+    const int ja = m / P.A1;
+    const int j1 = m - ja * P.A1;
+    const int N = P.N3;
+    const T kk = P.kx2[j1], kl = P.ky2[ja];
+    auto at = [&](int z) { return sidx(ja, z, j1, N, P.A1); };
+    if (-(kk + kl) > P.k2max) {
+        for (int z = 0; z < N; ++z) P.out[at(z)] = cscale(T(-1), P.in[at(z)]);
+        return;
+    }
+    const bool dir = P.dirichlet != 0;
+    const T coef = P.tnu / P.delta;
+    const T rs = T(-1) / P.delta;
+    const T adv = P.tadv * P.kx[j1] / P.delta;
+    const int lo = dir ? ZC_DZZD_LO : ZC_DZZN_LO;
+    // rows of (I - coef (Dzz - k^2) + i adv U) with the Neumannify/Dirichletify
+    // replacements of the implicit solves
+    auto rowc = [&](int j, T& a, V& b, T& c) {
+        a = -(coef * P.zc[lo * P.N3p + j]);
+        b.x = T(1) - coef * ((P.zc[(lo + 1) * P.N3p + j] + kk) + kl);
+        b.y = adv * P.zc[ZC_UBAR * P.N3p + j];
+        c = -(coef * P.zc[(lo + 2) * P.N3p + j]);
+        if (j == 0) a = 0;
+        if (j == N - 1) c = 0;
+        const bool id = dir ? (j <= 2 || j >= N - 2) : (j == 0 || j == N - 1);
+        if (id) {
+            a = 0;
+            b = mkc<T>(1, 0);
+            c = 0;
+        } else if (!dir && j == 1) {
+            a = 0;
+            b = mkc<T>(1, 0);
+            c = -1;
+        } else if (!dir && j == N - 2) {
+            a = 1;
+            b = mkc<T>(-1, 0);
+            c = 0;
+        }
+    };
+    auto rhs = [&](int j) {
+        const bool end = j <= 1 || j >= N - 2 || (dir && j == 2);
+        return end ? czero<V>() : cscale(rs, P.in[at(j)]);
+    };
+    auto cdiv = [](V a, V b) {
+        const T r = T(1) / (b.x * b.x + b.y * b.y);
+        return cscale(r, cmulc(a, b));
+    };
+    V cprev = czero<V>(), dprev = czero<V>();
+    for (int j = 0; j < N; ++j) {
+        T a, c;
+        V b;
+        rowc(j, a, b, c);
+        const V den = csub(b, cscale(a, cprev));
+        const V cj = cdiv(mkc<T>(c, 0), den);
+        const V dj = cdiv(csub(rhs(j), cscale(a, dprev)), den);
+        cp[j * ADV_T + t] = cj;
+        P.out[at(j)] = dj;
+        cprev = cj;
+        dprev = dj;
+    }
+    V x = dprev;
+    P.out[at(N - 1)] = csub(x, cscale(T(0.5), P.in[at(N - 1)]));
+    for (int j = N - 2; j >= 0; --j) {
+        x = csub(P.out[at(j)], cmul(cp[j * ADV_T + t], x));
+        P.out[at(j)] = csub(x, cscale(T(0.5), P.in[at(j)]));
+    }
+}
+
 template <typename T>
 hipError_t launch_diffprec(const DiffPrecArgs<T>& p, hipStream_t s) {
     const int M = (p.N3 + 63) / 64;
     const unsigned nb = unsigned(p.A1) * unsigned(p.A2);
     if (nb == 0 || p.in == p.out || !(p.delta > T(0))) return hipErrorInvalidValue;
+    if (p.tadv != T(0)) {
+        const size_t lds = size_t(ADV_T) * size_t(p.N3) * sizeof(cx<T>);
+        if (lds > 64 * 1024) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_advprec<T>), dim3((nb + ADV_T - 1) / ADV_T), dim3(ADV_T), lds, s, p);
+        return hipGetLastError();
+    }
     switch (M) {
 #define SF_M(m)                                                                   \
     case m:                                                                       \

@@ -2,7 +2,7 @@
 // preconditioner (include/stratiflow/DiffusionPreconditioner.h), on the
 // held-out problems of learned_demo.cpp (64x1x64, T = 100 steps of 0.02).
 // FP32 time stepping, FP64 Krylov algebra.
-//   argv: ntest K tol Re delta kmax
+//   argv: ntest K tol Re delta kmax advect
 // Prints, one per line:
 //   test <problem> <plain matvecs> <plain s> <precond matvecs> <precond s> <plain res> <precond res>
 //        <|dx_precond - dx_plain| / |dx_plain|>
@@ -56,11 +56,12 @@ int main(int argc, char** argv) {
     flowParams.Pr = 1;
     const double delta = argc > 5 ? std::atof(argv[5]) : 1e-3;
     const double kmax = argc > 6 ? std::atof(argv[6]) : 0.0;
+    const bool advect = argc > 7 && std::atoi(argv[7]) != 0;
     StateVector::ResetForParams();
     StateVector::solver().SetBackground(InitialU);
 
     typedef WithDiffusionPreconditioner<FixedNewton> Pre;
-    Pre nk(DiffusionPreconditioner(0.02 * 100, delta, kmax));
+    Pre nk(DiffusionPreconditioner(0.02 * 100, delta, kmax, advect));
     nk.verbose = false;
     nk.SetKrylovDimension(K);
     nk.gmresTolerance = tol;
