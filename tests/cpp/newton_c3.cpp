@@ -1,10 +1,12 @@
 // Newton-Krylov search for a steady state of the 2D stratified shear layer
 // (Newton-GMRES with FP64 Krylov algebra over FP32 or FP64 time stepping),
 // then the leading Arnoldi eigenvalues of the linearised map about it.
-//   newton_c3 <N1> <N3> <T> <plain steps> <amplitude> <K> <max Newton> <rel tol> <out dir> [Delta]
+//   newton_c3 <N1> <N3> <T> <plain steps> <amplitude> <K> <max Newton> <rel tol> <out dir> [Delta] [pdelta]
+// pdelta > 0: GMRES right-preconditioned by the advected diffusion preconditioner
+// (include/stratiflow/DiffusionPreconditioner.h) with that delta, on every mode.
 // Re=2000, Pr=1, Ri=0.12, U = tanh(z), B = tanh(z) (dB/dz = sech^2), dt = 0.01.
 // Prints:
-//   newton <step> <residual> <GMRES iterations> <seconds>
+//   newton <step> <residual> <GMRES iterations> <seconds> <GMRES relative residual>
 //   done <Newton steps> <final/initial residual> <seconds> <matvecs>
 //   ev <k> <re> <im>
 // and writes <out>/guess.fields and <out>/solution.fields.
@@ -16,6 +18,7 @@
 #include <vector>
 
 #include "stratiflow/Arnoldi.h"
+#include "stratiflow/DiffusionPreconditioner.h"
 #include "stratiflow/NewtonKrylov.h"
 
 static double now() {
@@ -90,6 +93,7 @@ int main(int argc, char** argv) {
     const double reltol = std::atof(argv[8]);
     const std::string dir = argv[9];
     const double Delta = argc > 10 ? std::atof(argv[10]) : 100.0;  // trust region (NewtonKrylov.h:29)
+    const double pdelta = argc > 11 ? std::atof(argv[11]) : 0.0;
     gridParams = {N1, 1, N3, Dimensionality::TwoDimensional};
     flowParams.L1 = stratifloat(2 * 3.14159265358979 / 0.44);
     flowParams.L2 = 1;
@@ -114,7 +118,8 @@ int main(int argc, char** argv) {
     x.SaveToFile(dir + "/guess");
     x.LoadFromFile(dir + "/guess.fields");
 
-    SteadyNewton nk;
+    WithDiffusionPreconditioner<SteadyNewton> nk(DiffusionPreconditioner(T, pdelta > 0 ? pdelta : 1.0, 1e9, true));
+    nk.enabled = pdelta > 0;
     nk.verbose = false;
     nk.targetResidual = 0;  // the relative tolerance below decides
     nk.Delta = Delta;
@@ -130,7 +135,7 @@ int main(int argc, char** argv) {
         nk.Run(x);  // one Newton step (evaluates the residual first)
         r = nk.residualHistory.front();
         if (r0 < 0) r0 = r;
-        printf("newton %d %.6e %d %.3f\n", it, r, nk.lastKrylovDimension, now() - t);
+        printf("newton %d %.6e %d %.3f %.3e\n", it, r, nk.lastKrylovDimension, now() - t, nk.lastGmresResidual);
         fflush(stdout);
         steps = it;
         if (r <= reltol * r0) break;
