@@ -5,7 +5,9 @@
 // LearnedPreconditioner wraps the library's network (sf_net_*): a few
 // Conv1D layers over the vertical profile of each horizontal mode, BF16 on the
 // matrix cores, trained on the GPU.  Apply(v) = v + s * net(v / s), s = RMS(v),
-// is the identity until the network is trained.
+// is the identity until the network is trained.  With an analytic base B
+// (SetBase, a DiffusionPreconditioner) the network learns a correction to it:
+// Apply(v) = B v + s * net(v / s), trained on the pairs (w, q - B w + w).
 //
 // WithLearnedPreconditioner<Solver> adds it to any NewtonKrylov<StateVector>
 // driver: while `harvest` is set, every GMRES operator application (x, A x)
@@ -18,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "DiffusionPreconditioner.h"
 #include "NewtonKrylov.h"
 #include "stratiflow.h"
 
@@ -38,16 +41,32 @@ public:
         sf_field* in[4] = {v.u1.dev(false), v.u2.dev(false), v.u3.dev(false), v.b.dev(false)};
         sf_field* o[4] = {out.u1.dev(true), out.u2.dev(true), out.u3.dev(true), out.b.dev(true)};
         check(sf_net_apply(net_, in, o));
+        if (base_) {
+            out -= v;
+            out += base_->Apply(v);
+        }
         out.EnforceBCs();
         return out;
     }
+    // analytic base preconditioner that the network corrects (nullptr: none);
+    // it must outlive this object
+    void SetBase(const DiffusionPreconditioner* base) { base_ = base; }
     // dilation of each layer's taps (1 = neighbouring points); dilations 1, 2,
     // 4, 8 let four layers see 61 vertical points instead of 17
     void SetDilations(const std::vector<int>& d) { check(sf_net_set_dilations(net_, d.data())); }
     // a pair with A q = w
     void AddPair(const StateVector& w, const StateVector& q) {
+        StateVector t;
+        const StateVector* target = &q;
+        if (base_) {  // the network's own output should be q - B w + w
+            t = q;
+            t -= base_->Apply(w);
+            t += w;
+            target = &t;
+        }
         sf_field* a[4] = {w.u1.dev(false), w.u2.dev(false), w.u3.dev(false), w.b.dev(false)};
-        sf_field* b[4] = {q.u1.dev(false), q.u2.dev(false), q.u3.dev(false), q.b.dev(false)};
+        sf_field* b[4] = {target->u1.dev(false), target->u2.dev(false), target->u3.dev(false),
+                          target->b.dev(false)};
         check(sf_net_add_pair(net_, a, b));
     }
     int Samples() const {
@@ -70,6 +89,7 @@ private:
     }
     sf_net* net_ = nullptr;
     bool trained_ = false;
+    const DiffusionPreconditioner* base_ = nullptr;
 };
 
 template <class Solver>

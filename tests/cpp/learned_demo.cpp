@@ -1,7 +1,9 @@
 // Trains the learned preconditioner (include/stratiflow/LearnedPreconditioner.h)
 // on the operator applications of plain GMRES solves, then solves held-out
 // problems with and without it (argv[7] > 0: dilated layers 1, 2, 4, 8;
-// argv[8]: rounds of retraining on the directions of preconditioned solves).
+// argv[8]: rounds of retraining on the directions of preconditioned solves;
+// argv[9] > 0: the network corrects the advected diffusion preconditioner with
+// that delta instead of the identity).
 // FP32 time stepping (the facade's default
 // precision), FP64 Krylov algebra.  Prints, one per line:
 //   train <problem> <gmres iterations> <seconds>
@@ -68,6 +70,9 @@ int main(int argc, char** argv) {
 
     LearnedPreconditioner pre(64, 4, 1);
     if (argc > 7 && std::atoi(argv[7]) > 0) pre.SetDilations({1, 2, 4, 8});
+    const double bdelta = argc > 9 ? std::atof(argv[9]) : 0.0;
+    const DiffusionPreconditioner base(0.02 * 100, bdelta > 0 ? bdelta : 1.0, 1e9, true);
+    if (bdelta > 0) pre.SetBase(&base);
     Learned nk(pre);
     nk.verbose = false;
     nk.SetKrylovDimension(K);
