@@ -38,3 +38,22 @@ def test_learned_preconditioner_judged_by_plain_gmres():
         # the judged update equals the plain one: exactly after a fallback,
         # within the GMRES tolerance when the preconditioned solve converged
         assert t["diff"] <= (0.0 if t["learned_res"] >= 1e-2 else 0.05), t
+
+
+@pytest.mark.gpu
+def test_network_correcting_the_analytic_base():
+    """With the advected diffusion preconditioner as its base (learned_demo's
+    ninth argument) the trained network's preconditioned solves converge on
+    held-out problems without falling back and with fewer operator
+    applications than plain GMRES."""
+    r = subprocess.run([BIN, "3", "2", "61", "2e-2", "40", "100", "1", "0", "0.3"], capture_output=True, text=True,
+                       timeout=280)
+    assert r.returncode == 0, r.stdout + r.stderr
+    tests = [line.split() for line in r.stdout.splitlines() if line.startswith("test")]
+    assert len(tests) == 2
+    for t in tests:
+        plain_mv, learned_mv, learned_res, fallbacks = int(t[2]), int(t[4]), float(t[7]), int(t[9])
+        print("plain %d, learned with base %d (res %.2e, fallbacks %d)" % (plain_mv, learned_mv, learned_res,
+                                                                         fallbacks), flush=True)
+        assert fallbacks == 0 and learned_res < 2e-2
+        assert learned_mv < plain_mv
