@@ -2,11 +2,16 @@
 // preconditioner (include/stratiflow/DiffusionPreconditioner.h), on the
 // held-out problems of learned_demo.cpp (64x1x64, T = 100 steps of 0.02).
 // FP32 time stepping, FP64 Krylov algebra.
-//   argv: ntest K tol Re delta kmax advect
+//   argv: ntest K tol Re delta kmax advect [newton]
 // Prints, one per line:
 //   test <problem> <plain matvecs> <plain s> <precond matvecs> <precond s> <plain res> <precond res>
 //        <|dx_precond - dx_plain| / |dx_plain|>
 //   total <plain matvecs> <precond matvecs> <plain converged> <precond converged>
+// With newton > 0, full Newton iterations instead (at most `newton` steps, GMRES
+// tolerance tol per step) until |F(x)| <= 1e-8 |F(x0)|:
+//   newton <problem> <plain steps> <plain matvecs> <plain s> <plain |F|/|F0|>
+//          <precond steps> <precond matvecs> <precond s> <precond |F|/|F0|>
+//   newton_total <plain s> <precond s> <plain converged> <precond converged>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -66,6 +71,49 @@ int main(int argc, char** argv) {
     nk.SetKrylovDimension(K);
     nk.gmresTolerance = tol;
     nk.maxNewtonSteps = 1;
+
+    const int newton = argc > 8 ? std::atoi(argv[8]) : 0;
+    if (newton > 0) {
+        double sp = 0, sl = 0;
+        int okp = 0, okl = 0;
+        for (int i = 0; i < ntest; ++i) {
+            const StateVector x0 = problem(1000 + i);
+            double r0 = 0;
+            {
+                StateVector y = x0;
+                FixedNewton probe;
+                probe.verbose = false;
+                probe.SetKrylovDimension(2);
+                probe.maxNewtonSteps = 0;
+                probe.Run(y);  // residual evaluation only
+                r0 = probe.residualHistory.front();
+            }
+            nk.targetResidual = 1e-8 * r0;
+            nk.maxNewtonSteps = newton;
+            double res[2], sec[2];
+            int steps[2];
+            long mv[2];
+            for (int pass = 0; pass < 2; ++pass) {
+                StateVector x = x0;
+                nk.enabled = pass == 1;
+                const long m0 = nk.matVecs;
+                const double t = now();
+                nk.Run(x);
+                sec[pass] = now() - t;
+                mv[pass] = nk.matVecs - m0;
+                steps[pass] = int(nk.residualHistory.size()) - 1;
+                res[pass] = nk.residualHistory.back() / r0;
+            }
+            printf("newton %d %d %ld %.4f %.3e %d %ld %.4f %.3e\n", i, steps[0], mv[0], sec[0], res[0], steps[1], mv[1],
+                   sec[1], res[1]);
+            sp += sec[0];
+            sl += sec[1];
+            okp += res[0] <= 1e-8;
+            okl += res[1] <= 1e-8;
+        }
+        printf("newton_total %.4f %.4f %d %d\n", sp, sl, okp, okl);
+        return 0;
+    }
 
     long tp_mv = 0, tl_mv = 0;
     int cp = 0, cl = 0;
