@@ -3,7 +3,10 @@
 // problems with and without it (argv[7] > 0: dilated layers 1, 2, 4, 8;
 // argv[8]: rounds of retraining on the directions of preconditioned solves;
 // argv[9] > 0: the network corrects the advected diffusion preconditioner with
-// that delta instead of the identity).
+// that delta instead of the identity; argv[10] = n > 0: training solves and
+// tests are full Newton runs of up to n steps, the tests to |F| <= 1e-8 |F(x0)|,
+// printed as  newton <problem> <plain steps> <plain matvecs> <plain s> <plain |F|/|F0|>
+// <learned steps> <learned matvecs> <learned s> <learned |F|/|F0|>).
 // FP32 time stepping (the facade's default
 // precision), FP64 Krylov algebra.  Prints, one per line:
 //   train <problem> <gmres iterations> <seconds>
@@ -78,6 +81,12 @@ int main(int argc, char** argv) {
     nk.SetKrylovDimension(K);
     nk.gmresTolerance = tol;
     nk.maxNewtonSteps = 1;
+    const int newton = argc > 10 ? std::atoi(argv[10]) : 0;
+    if (newton > 0) {
+        nk.maxNewtonSteps = newton;
+        nk.targetResidual = 0;
+        nk.fallback = false;
+    }
 
     // harvest (A x, x) from plain solves
     nk.harvest = true;
@@ -109,6 +118,39 @@ int main(int argc, char** argv) {
         printf("loss %.6e %.6e %d %.3f\n", l0, l1, pre.Samples(), now() - t1);
     }
 
+    if (newton > 0) {
+        for (int i = 0; i < ntest; ++i) {
+            const StateVector x0 = problem(1000 + i);
+            double r0 = 0;
+            {
+                StateVector y = x0;
+                FixedNewton probe;
+                probe.verbose = false;
+                probe.SetKrylovDimension(2);
+                probe.maxNewtonSteps = 0;
+                probe.Run(y);  // residual evaluation only
+                r0 = probe.residualHistory.front();
+            }
+            nk.targetResidual = 1e-8 * r0;
+            double res[2], sec[2];
+            int steps[2];
+            long mv[2];
+            for (int pass = 0; pass < 2; ++pass) {
+                StateVector x = x0;
+                nk.enabled = pass == 1;
+                const long m0 = nk.matVecs;
+                const double t = now();
+                nk.Run(x);
+                sec[pass] = now() - t;
+                mv[pass] = nk.matVecs - m0;
+                steps[pass] = int(nk.residualHistory.size()) - 1;
+                res[pass] = nk.residualHistory.back() / r0;
+            }
+            printf("newton %d %d %ld %.4f %.3e %d %ld %.4f %.3e\n", i, steps[0], mv[0], sec[0], res[0], steps[1], mv[1],
+                   sec[1], res[1]);
+        }
+        return 0;
+    }
     for (int i = 0; i < ntest; ++i) {
         const StateVector x0 = problem(1000 + i);
         StateVector xp = x0, xl = x0;
