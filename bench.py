@@ -12,7 +12,11 @@ kernel's algorithmic bytes per launch over its average launch time;
 ``step_roofline`` applies the per-step byte model to the whole step;
 ``cpu_baseline`` times the CPU oracle (a C++ restatement of the reference,
 OpenMP) on the host cores for a bounded sample of the same workload.
+``newton_gmres`` (after the timed region, from tests/cpp/precond_demo when it
+is built) compares plain and preconditioned Newton-GMRES solves on the 8
+held-out 64x1x64 problems: operator applications, wall time, problems solved.
 """
+import subprocess
 import argparse
 import json
 import os
@@ -154,12 +158,44 @@ def ensemble(sf, ic, nruns=64, steps=500, seed=20261202):
     return out
 
 
+def newton_gmres(timeout=240):
+    """One GMRES solve per problem of F(x) = Phi_2(x) - x on the 8 held-out 64x1x64
+    problems (Re = 100, Krylov limit 60, tolerance 2e-2), plain and right-
+    preconditioned by the advected diffusion preconditioner (delta = 0.3);
+    None when the demo program has not been built."""
+    binary = os.path.join(HERE, "tests", "cpp", "_build", "precond_demo")
+    if not os.path.exists(binary):
+        return None
+    r = subprocess.run([binary, "8", "61", "2e-2", "100", "0.3", "100", "1"], capture_output=True, text=True,
+                       timeout=timeout)
+    if r.returncode != 0:
+        return {"error": (r.stdout + r.stderr)[-400:]}
+    rows = [line.split() for line in r.stdout.splitlines() if line.startswith("test ")]
+    tol = 2e-2
+    plain_s = sum(float(t[3]) for t in rows)
+    pre_s = sum(float(t[5]) for t in rows)
+    both = [t for t in rows if float(t[6]) < tol and float(t[7]) < tol]
+    return {
+        "workload": "one Newton-GMRES solve per problem, 8 held-out problems, 64x1x64 2D, Re=100, T=2, K=60, "
+                    "tol 2e-2; preconditioner (T L_k - 0.3)^-1 - 1/2, L_k = nu(Dzz-k^2) - i kx U(z)",
+        "plain_matvecs": sum(int(t[2]) for t in rows),
+        "precond_matvecs": sum(int(t[4]) for t in rows),
+        "plain_s": plain_s,
+        "precond_s": pre_s,
+        "plain_solved": sum(float(t[6]) < tol for t in rows),
+        "precond_solved": sum(float(t[7]) < tol for t in rows),
+        "time_ratio_where_both_solve": (sum(float(t[3]) for t in both) / sum(float(t[5]) for t in both))
+        if both else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-newton", action="store_true", help="skip the Newton-GMRES comparison")
     ap.add_argument("--cpu-seconds", type=float, default=40.0, help="budget for the CPU baseline sample")
     ap.add_argument("--traffic-json", default=os.path.join(HERE, "profiles", "traffic_384x192x384_fp32.json"))
     ap.add_argument("--ensemble", action="store_true",
@@ -247,6 +283,9 @@ def main():
     }
     if args.ensemble:
         out["ensemble"] = ensemble(sf, ic)
+    if not args.no_newton:
+        s.close()
+        out["newton_gmres"] = newton_gmres()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(U, S, pert, args.cpu_seconds)
     print(json.dumps(out), flush=True)
