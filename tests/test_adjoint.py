@@ -286,7 +286,8 @@ def _gpu_adjoint(N1, N2, N3, dim, prec, nsteps, dt=0.02):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dim,N1,N2,N3", [(O.TWO_D, 48, 1, 64), (O.THREE_D, 24, 12, 32)])
+@pytest.mark.parametrize("dim,N1,N2,N3", [(O.TWO_D, 48, 1, 64), (O.THREE_D, 24, 12, 32), (O.TWO_D, 64, 1, 100),
+                                          (O.THREE_D, 12, 6, 36)])
 @pytest.mark.parametrize("prec,tol", [(8, 1e-12), (4, 2e-4)])
 def test_gpu_adjoint_step_matches_oracle(dim, N1, N2, N3, prec, tol):
     print("adjoint parity dim=%d %dx%dx%d prec=%d" % (dim, N1, N2, N3, prec), flush=True)
