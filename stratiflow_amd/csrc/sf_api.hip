@@ -1426,7 +1426,7 @@ int sf_field_diffusion_precondition(const sf_field* in, sf_field* out, double tn
             p.delta = T(delta);
             p.k2max = kmax < 0 ? T(-1) : T(kmax * kmax);
             p.tadv = T(tadv);
-            SF_REQUIRE(tadv == 0 || size_t(C.N3) * 16 * sizeof(cx<T>) <= 64 * 1024, SF_ERR_UNSUPPORTED,
+            SF_REQUIRE(tadv == 0 || size_t(C.N3) * ADVPREC_MODES * sizeof(cx<T>) <= ADVPREC_LDS, SF_ERR_UNSUPPORTED,
                        "the advected preconditioner supports N3 <= 512 (FP32) or 256 (FP64)");
             C.check(launch_diffprec<T>(p, C.stream), "diffusion preconditioner");
         });

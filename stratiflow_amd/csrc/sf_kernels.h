@@ -116,6 +116,10 @@ struct DiffPrecArgs {
 };
 template <typename T>
 hipError_t launch_diffprec(const DiffPrecArgs<T>& p, hipStream_t s);
+// the advected variant runs ADVPREC_MODES modes per block, each with N3 complex
+// values of LDS, within ADVPREC_LDS bytes
+constexpr int ADVPREC_MODES = 16;
+constexpr size_t ADVPREC_LDS = 64 * 1024;
 
 // adjoint step (sf_adjoint.hip)
 template <typename T>

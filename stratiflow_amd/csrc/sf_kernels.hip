@@ -1444,7 +1444,7 @@ __global__ __launch_bounds__(64) void k_diffprec(DiffPrecArgs<T> P) {
 // runs a Thomas sweep (the system is strictly diagonally dominant for
 // delta > 0) with its modified super-diagonal in LDS and the modified
 // right-hand side in out; ADV_T modes per block.
-constexpr int ADV_T = 16;
+constexpr int ADV_T = ADVPREC_MODES;
 template <typename T>
 __global__ __launch_bounds__(ADV_T) void k_advprec(DiffPrecArgs<T> P) {
     using V = cx<T>;
@@ -1527,7 +1527,7 @@ hipError_t launch_diffprec(const DiffPrecArgs<T>& p, hipStream_t s) {
     if (nb == 0 || p.in == p.out || !(p.delta > T(0))) return hipErrorInvalidValue;
     if (p.tadv != T(0)) {
         const size_t lds = size_t(ADV_T) * size_t(p.N3) * sizeof(cx<T>);
-        if (lds > 64 * 1024) return hipErrorInvalidValue;
+        if (lds > ADVPREC_LDS) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_advprec<T>), dim3((nb + ADV_T - 1) / ADV_T), dim3(ADV_T), lds, s, p);
         return hipGetLastError();
     }
