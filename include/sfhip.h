@@ -215,6 +215,12 @@ int sf_remove_divergence(sf_ctx* ctx, double pressure_multiplier);
 int sf_filter_all(sf_ctx* ctx);
 /* IMEXRK::PopulateNodalVariables (IMEXRK.h:141-162) */
 int sf_populate_nodal(sf_ctx* ctx);
+/* The nodal fields SF_NODAL_U1..SF_NODAL_B as the reference holds them at
+ * this point (what IMEXRK::SaveFlow writes, IMEXRK.h:415-427): the step never
+ * writes physical-space fields, so they are transformed on request from the
+ * spectra of the last sf_populate_nodal while it is pending (before the next
+ * stage), else from the state that the last stage left. */
+int sf_refresh_nodal(sf_ctx* ctx);
 /* IMEXRK::KE and PE (IMEXRK.h:395-411), FP64 reductions */
 int sf_energy(sf_ctx* ctx, double* ke, double* pe);
 /* KE and PE under the name of their FP64 reductions (same as sf_energy) */
@@ -222,6 +228,10 @@ int sf_energy_f64(sf_ctx* ctx, double* ke, double* pe);
 /* IMEXRK::CFL (IMEXRK.h:355-374) without the dt update: returns the CFL
  * number for the current dt from the nodal fields (populates them) */
 int sf_cfl(sf_ctx* ctx, double* cfl);
+/* IMEXRK::CFLlinear (IMEXRK.h:376-393) without the dt update: the CFL number
+ * for the current dt of the linearised step, from the state set with
+ * sf_set_linear_background and 1 as the bound on |U_| */
+int sf_cfl_linear(sf_ctx* ctx, double* cfl);
 int sf_synchronize(sf_ctx* ctx);
 /* nsteps of sf_step bracketed by HIP events on the context's stream;
  * *ms = elapsed milliseconds */

@@ -73,6 +73,8 @@ def lib():
         L.sfo_prepare_adjoint.argtypes = [vp]
         L.sfo_jover_k.argtypes = [vp]
         L.sfo_jover_k.restype = d
+        L.sfo_cfl.argtypes = [vp, i]
+        L.sfo_cfl.restype = d
         L.sfo_adjoint.argtypes = [vp, i, ctypes.POINTER(dp), ctypes.POINTER(dp), d]
         _lib = L
     return _lib
@@ -97,6 +99,7 @@ class Oracle:
         self.h = lib().sfo_create(N1, N2, N3, dim, L1, L2, L3, Re, Ri, Pr, precision)
         if not self.h:
             raise RuntimeError("sfo_create failed")
+        self.dt = 0.01  # the solver's initial deltaT
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -112,6 +115,7 @@ class Oracle:
         return (self.N2, self.N1, self.N3)
 
     def set_dt(self, dt):
+        self.dt = float(dt)
         lib().sfo_set_dt(self.h, dt)
 
     def set_flow(self, Re, Ri, Pr):
@@ -169,6 +173,12 @@ class Oracle:
     def jover_k(self):
         """IMEXRK::JoverK (IMEXRK.h:445-459) from the nodal B and the modal u3."""
         return lib().sfo_jover_k(self.h)
+
+    def cfl(self, linear=False):
+        """CFL (IMEXRK.h:355-374) or CFLlinear (IMEXRK.h:376-393) for the current dt,
+        without their dt update; CFL reads the nodal U1..U3 as they are, CFLlinear
+        the nodal linearisation state of set_linear_background."""
+        return lib().sfo_cfl(self.h, int(bool(linear)))
 
     def prepare_adjoint(self):
         """PrepareRunAdjoint (IMEXRK.h:183-199): zero pressure, populate nodal fields."""

@@ -1171,6 +1171,45 @@ public:
         return J / K;
     }
 
+    // NodalField::Max (Field.h:771-786): the largest |value| over every z level
+    // (ghost levels included) and every horizontal point; `add` is a profile
+    // broadcast over the stacks first (U1 + U_ of IMEXRK.h:359)
+    T nodal_max(const std::vector<T>& f, const std::vector<T>* add) const {
+        T m = 0;
+        for (size_t o = 0; o < nn; o += size_t(N3))
+            for (int j = 0; j < N3; ++j) {
+                const T v = add ? T(f[o + j] + (*add)[j]) : f[o + j];
+                m = std::max(m, std::abs(v));
+            }
+        return m;
+    }
+    // the three grid spacings of CFL/CFLlinear (IMEXRK.h:359-364, 380-384):
+    // L1/N1, L2/N2 and the smallest vertical gap z(N3/2+1) - z(N3/2)
+    void cfl_spacings(T& d1, T& d2, T& d3) const {
+        d1 = T(L1) / T(N1);
+        d2 = T(L2) / T(N2);
+        d3 = T(g.z[N3 / 2 + 1]) - T(g.z[N3 / 2]);
+    }
+    // IMEXRK::CFL (IMEXRK.h:355-374) before its dt update:
+    //   dt * (max|U1 + U_|/d1 + max|U2|/d2 + max|U3|/d3)
+    // from the nodal fields as they are (U2 stays zero in 2D)
+    T cfl() const {
+        T d1, d2, d3;
+        cfl_spacings(d1, d2, d3);
+        T c = nodal_max(U1, &Ubar) / d1 + nodal_max(U2, nullptr) / d2 + nodal_max(U3, nullptr) / d3;
+        return c * T(deltaT);
+    }
+    // IMEXRK::CFLlinear (IMEXRK.h:376-393) before its dt update: the nodal
+    // linearisation state of SetBackground(u1..b) (IMEXRK.h:334-352), with 1 as
+    // the bound on |U_|:  dt * ((1 + max|U1_tot|)/d1 + max|U2_tot|/d2 + max|U3_tot|/d3)
+    T cfl_linear() const {
+        T d1, d2, d3;
+        cfl_spacings(d1, d2, d3);
+        T c = (T(1) + nodal_max(U1_tot, nullptr)) / d1 + nodal_max(U2_tot, nullptr) / d2 +
+              nodal_max(U3_tot, nullptr) / d3;
+        return c * T(deltaT);
+    }
+
     // PrepareRunAdjoint, IMEXRK.h:183-199 (without the image folders)
     void prepare_adjoint() {
         zero(p);
@@ -1429,6 +1468,11 @@ void sfo_adjoint(void* p, int mode, const double* const* below, const double* co
 }
 double sfo_jover_k(void* p) {
     return dispatch(static_cast<Handle*>(p), [&](auto& s) { return double(s.jover_k()); });
+}
+// CFL (linear == 0, IMEXRK.h:355-374) or CFLlinear (IMEXRK.h:376-393) for the
+// current dt, without its dt update
+double sfo_cfl(void* p, int linear) {
+    return dispatch(static_cast<Handle*>(p), [&](auto& s) { return double(linear ? s.cfl_linear() : s.cfl()); });
 }
 void sfo_energy(void* p, double* ke, double* pe) {
     dispatch(static_cast<Handle*>(p), [&](auto& s) { *ke = s.ke(); *pe = s.pe(); return 0; });

@@ -59,7 +59,7 @@ EXPORTED = [
     "sf_set_linear_background", "sf_step_adjoint", "sf_jover_k", "sf_field_add_profile", "sf_field_mean_profile",
     "sf_dot_f64", "sf_field_enforce_bcs", "sf_phase_shift", "sf_remove_average", "sf_state_bind", "sf_state_fetch",
     "sf_energy_f64", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
-    "sf_energy", "sf_cfl", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
+    "sf_energy", "sf_cfl", "sf_cfl_linear", "sf_refresh_nodal", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
     "sf_batch_create", "sf_batch_destroy", "sf_batch_last_error", "sf_batch_step", "sf_batch_time_steps",
     "sf_batch_info",
     "sf_net_create", "sf_net_destroy", "sf_net_last_error", "sf_net_apply", "sf_net_add_pair", "sf_net_clear_pairs",
@@ -128,6 +128,8 @@ def lib():
         "sf_populate_nodal": [vp],
         "sf_energy": [vp, dp, dp],
         "sf_cfl": [vp, dp],
+        "sf_cfl_linear": [vp, dp],
+        "sf_refresh_nodal": [vp],
         "sf_synchronize": [vp],
         "sf_time_steps": [vp, i, dp],
         "sf_time_steps_detailed": [vp, i, dp, dp, ctypes.POINTER(ctypes.c_int)],
@@ -457,6 +459,32 @@ class IMEXRK:
         out = ctypes.c_double()
         self._chk(lib().sf_inner_prod(a.h, b.h, ctypes.byref(out)))
         return out.value
+
+    def RefreshNodal(self):
+        """Materialise the nodal fields U1..B as the reference holds them now."""
+        self._chk(lib().sf_refresh_nodal(self.h))
+
+    def cfl_number(self, linear=False):
+        """CFL (IMEXRK.h:355-374) or CFLlinear (IMEXRK.h:376-393) for the current
+        deltaT, without changing it."""
+        out = ctypes.c_double()
+        fn = lib().sf_cfl_linear if linear else lib().sf_cfl
+        self._chk(fn(self.h, ctypes.byref(out)))
+        return out.value
+
+    def CFL(self):
+        """IMEXRK::CFL (IMEXRK.h:355-374): returns the CFL number for the current
+        deltaT and rescales deltaT to the target CFL 0.8."""
+        cfl = self.cfl_number(False)
+        self.deltaT = self.deltaT * 0.8 / cfl
+        return cfl
+
+    def CFLlinear(self):
+        """IMEXRK::CFLlinear (IMEXRK.h:376-393): as CFL with target 0.4, from the
+        linearisation state."""
+        cfl = self.cfl_number(True)
+        self.deltaT = self.deltaT * 0.4 / cfl
+        return cfl
 
     def synchronize(self):
         self._chk(lib().sf_synchronize(self.h))

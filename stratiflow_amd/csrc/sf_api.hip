@@ -585,6 +585,17 @@ struct Ctx {
         to_nodal(mf(SF_B), static_cast<T*>(state[SF_NODAL_B].d));
     }
 
+    // the nodal fields U1..B as the reference holds them between calls: the
+    // transforms of the last PopulateNodalVariables while its snapshot is
+    // pending, else of the state (every stage ends with PopulateNodalVariables,
+    // IMEXRK.cpp:8-24).  The step itself never writes them.
+    void refresh_nodal() {
+        for (int i = 0; i < 4; ++i) {
+            if (i == 1 && !three) continue;
+            to_nodal(src(i), static_cast<T*>(state[SF_NODAL_U1 + i].d));
+        }
+    }
+
     // ---- adjoint step (IMEXRK.h:86-126) ----
     // nodal: direct totals, their horizontal derivatives, explicit terms;
     // modal: transformed explicit terms and one derivative scratch field.
@@ -771,6 +782,25 @@ struct Ctx {
         read_scalars(3, m);
         if (!three) m[1] = 0;
         return (m[0] / d1 + m[1] / d2 + m[2] / d3) * dt;
+    }
+
+    // IMEXRK::CFLlinear (IMEXRK.h:376-393) for the current dt, without changing
+    // it: dt*((1 + max|U1_tot|)/d1 + max|U2_tot|/d2 + max|U3_tot|/d3) over the
+    // nodal linearisation state (SetBackground(u1..b), IMEXRK.h:334-352); 1
+    // bounds |U_|.  The nodal totals go through one scratch field in turn.
+    T* ntmp = nullptr;
+    double cfl_linear() {
+        if (!ntmp) ntmp = dalloc<T>(nnodal);
+        const VGrid g = vertical_grid(L3, N3);
+        const double d1 = L1 / N1, d2 = L2 / N2, d3 = g.z[N3 / 2 + 1] - g.z[N3 / 2];
+        double m[3] = {0, 0, 0};
+        for (int i = 0; i < 3; ++i) {
+            if (i == 1 && !three) continue;
+            to_nodal(mf(SF_U1_TOT + i), ntmp);
+            check(launch_maxabs<T>(ntmp, nnodal, N3, nullptr, partial, scal, 0, stream), "cfl_linear");
+            read_scalars(1, &m[i]);
+        }
+        return ((1.0 + m[0]) / d1 + m[1] / d2 + m[2] / d3) * dt;
     }
 
     void energy(double* ke, double* pe) {
@@ -1649,6 +1679,17 @@ int sf_cfl(sf_ctx* ctx, double* cfl) {
     SF_NULLCHK(ctx);
     SF_NULLCHK(cfl);
     return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { *cfl = C.cfl(); }); });
+}
+
+int sf_refresh_nodal(sf_ctx* ctx) {
+    SF_NULLCHK(ctx);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { C.refresh_nodal(); }); });
+}
+
+int sf_cfl_linear(sf_ctx* ctx, double* cfl) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(cfl);
+    return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { *cfl = C.cfl_linear(); }); });
 }
 
 int sf_synchronize(sf_ctx* ctx) {

@@ -42,6 +42,101 @@ def test_grid_closed_form(N):
     assert DYF[N // 2] < DYF[2]
 
 
+@pytest.mark.parametrize("N", [16, 64, 384])
+def test_grid_is_the_degree7_map(N):
+    """z_1..z_{N-1} over x = linspace(-1, 1, N-1) is an odd degree-7 polynomial
+    s*(2x^7 + x)/3 (Field.cpp:9-13): a least-squares fit recovers the x^7 : x
+    coefficient ratio 2 and nothing else.  The older cubic map of Field.cpp:33-48
+    (2x^3 + x)/3 and a dropped or changed power would put weight on other
+    coefficients or change the ratio."""
+    z, zf, DY, DYF = make(N3=N).grid()
+    x = np.linspace(-1.0, 1.0, N - 1)
+    c = np.polynomial.polynomial.polyfit(x, z[1:], 7)
+    s = 3 * c[1]
+    assert abs(c[7] / c[1] - 2.0) < 1e-9
+    for k in (0, 2, 3, 4, 5, 6):
+        assert abs(c[k]) < 1e-9 * abs(s)
+    # the scale puts the first Neumann point (z_1 + z_2)/2 on the wall -L
+    # (Field.cpp:15-17)
+    assert 0.5 * (z[1] + z[2]) == pytest.approx(-10.0, abs=1e-12)
+
+
+def test_grid_spacing_ratio_centre_to_wall():
+    """dz/dx of s*(2x^7 + x)/3 is 1 at the centre and 15 at the walls, so the
+    centre-to-wall spacing ratio tends to 1/15 as N grows (the cubic map of
+    Field.cpp:33-48 gives 1/7)."""
+    ratios = []
+    for N in (128, 512, 2048):
+        z, zf, DY, DYF = make(N3=N).grid()
+        ratios.append(DYF[N // 2] / DYF[1])
+    assert abs(ratios[-1] - 1.0 / 15.0) < 2e-3
+    assert abs(ratios[-1] - 1.0 / 15.0) < abs(ratios[0] - 1.0 / 15.0)
+
+
+# ---------------------------------------------------------------- CFL
+def _centre_gap(N, L=10.0):
+    """z(N/2+1) - z(N/2) of the x^7 map in closed form: the points sit at
+    x = 0 and x = 2/(N-2) of linspace(-1, 1, N-1), scaled so that the first
+    Neumann point lies on -L (Field.cpp:3-19)."""
+    h = lambda x: L * (2 * x ** 7 + x) / 3
+    x1, x2 = -1.0, -1.0 + 2.0 / (N - 2)
+    scale = -L / (0.5 * (h(x1) + h(x2)))
+    return scale * (h(2.0 / (N - 2)) - h(0.0))
+
+
+@pytest.mark.parametrize("dim,N2", [(O.TWO_D, 1), (O.THREE_D, 12)])
+def test_cfl_uniform_flow_closed_form(dim, N2):
+    """IMEXRK::CFL (IMEXRK.h:355-374) for uniform nodal velocities U1 = c,
+    U2 = v, U3 = w and a uniform background U_ = a:
+      dt * (|c + a| N1/L1 + |v| N2/L2 + |w| / (z(N3/2+1) - z(N3/2)))."""
+    N1, N3, dt = 24, 48, 0.013
+    o = make(N1=N1, N2=N2, N3=N3, dim=dim)
+    o.set_dt(dt)
+    c, a, v, w = 0.5, 0.25, -0.7, -0.3
+    o.set_background(np.full(N3, a))
+    shape = o.nodal_shape
+    o.set_nodal("U1", np.full(shape, c))
+    o.set_nodal("U3", np.full(shape, w))
+    if dim == O.THREE_D:
+        o.set_nodal("U2", np.full(shape, v))
+    d3 = _centre_gap(N3)
+    want = dt * (abs(c + a) * N1 / LX + (abs(v) * N2 / (LX / 2) if dim == O.THREE_D else 0.0) + abs(w) / d3)
+    assert o.cfl() == pytest.approx(want, rel=1e-12)
+    # the largest |value| wins, wherever it sits (here a single negative spike)
+    U3 = np.full(shape, w)
+    U3[0, N1 // 3, 5] = -2.0
+    o.set_nodal("U3", U3)
+    want2 = want + dt * (2.0 - abs(w)) / d3
+    assert o.cfl() == pytest.approx(want2, rel=1e-12)
+
+
+@pytest.mark.parametrize("dim,N2", [(O.TWO_D, 1), (O.THREE_D, 12)])
+def test_cfl_linear_uniform_closed_form(dim, N2):
+    """IMEXRK::CFLlinear (IMEXRK.h:376-393) about a uniform linearisation state
+    u1_tot = c, u2_tot = v, u3_tot = w (the (0,0) mode, so every nodal point
+    holds the value):  dt * ((1 + |c|) N1/L1 + |v| N2/L2 + |w|/dz_mid); the
+    background U_ does not enter (1 bounds it)."""
+    N1, N3, dt = 24, 48, 0.02
+    o = make(N1=N1, N2=N2, N3=N3, dim=dim)
+    o.set_dt(dt)
+    o.set_background(np.full(N3, 5.0))  # must not matter
+    c, v, w = -0.4, 0.6, 0.35
+
+    def uniform(val):
+        m = np.zeros(o.modal_shape, dtype=complex)
+        m[0, 0, :] = val
+        return m
+    o.set_modal("u1t", uniform(c))
+    o.set_modal("u3t", uniform(w))
+    o.set_modal("bt", uniform(0.0))
+    if dim == O.THREE_D:
+        o.set_modal("u2t", uniform(v))
+    o.set_linear_background()
+    d3 = _centre_gap(N3)
+    want = dt * ((1 + abs(c)) * N1 / LX + (abs(v) * N2 / (LX / 2) if dim == O.THREE_D else 0.0) + abs(w) / d3)
+    assert o.cfl(linear=True) == pytest.approx(want, rel=1e-12)
+
+
 # ---------------------------------------------------------------- operators
 def test_operators_exact_on_linear_data():
     o = make(N3=48)
