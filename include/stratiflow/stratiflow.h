@@ -14,17 +14,27 @@
 //    status codes; the reference asserts or throws std::string).
 #pragma once
 
+// The standard headers the reference's StateVector.h makes visible to its
+// callers through IMEXRK.h:12-22 and OSUtils.cpp (streams, directory listing,
+// assert), so that drivers written against it compile unchanged.
+#include <dirent.h>
+#include <sys/stat.h>
+
 #include <algorithm>
+#include <cassert>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <iomanip>
+#include <iostream>
+#include <map>
 #include <memory>
+#include <random>
 #include <stdexcept>
 #include <string>
-#include <random>
-#include <sys/stat.h>
 #include <vector>
 
 #include "../sfhip.h"
@@ -157,6 +167,121 @@ inline sf_ctx* ctx() {
 
 static GridParams& gridParams = sfacade::Globals<>::grid;
 static FlowParams& flowParams = sfacade::Globals<>::flow;
+
+// ---------------------------------------------------------------------------
+// Scalar files and parameter files (OSUtils.cpp:72-95, Parameters.cpp:55-111)
+// ---------------------------------------------------------------------------
+inline bool FileExists(const std::string& filename) {  // OSUtils.cpp:78-82
+    std::ifstream f(filename, std::ios::in | std::ios::binary);
+    return f.good();
+}
+inline bool EndsWith(const std::string& value, const std::string& ending) {  // OSUtils.h:24-28
+    return ending.size() <= value.size() && value.compare(value.size() - ending.size(), ending.size(), ending) == 0;
+}
+// a .params file: the value as text at 30 significant digits
+inline void SaveValueToFile(stratifloat value, const std::string& filename) {
+    std::ofstream out(filename);
+    out << std::setprecision(30) << value;
+}
+inline void LoadValueFromFile(stratifloat& value, const std::string& filename) {
+    std::ifstream in(filename);
+    in >> value;
+}
+
+namespace sfacade {
+// The lines of params.dat: each parameter's C++ expression as the name, then
+// its value (booleans as 0/1).  Readers match a line by its leading name.
+struct ParamLine {
+    std::string name;
+    double value;
+    bool integral;
+};
+inline std::vector<ParamLine> param_lines(bool twoAndAHalfFlag) {
+    const GridParams& g = Globals<>::grid;
+    const FlowParams& f = Globals<>::flow;
+    std::vector<ParamLine> v = {
+        {"gridParams.N1", double(g.N1), true},
+        {"gridParams.N2", double(g.N2), true},
+        {"gridParams.N3", double(g.N3), true},
+        {"(gridParams.dimensionality == Dimensionality::ThreeDimensional)",
+         double(g.dimensionality == Dimensionality::ThreeDimensional), true}};
+    if (twoAndAHalfFlag)
+        v.push_back({"(gridParams.dimensionality == Dimensionality::TwoAndAHalf)",
+                     double(g.dimensionality == Dimensionality::TwoAndAHalf), true});
+    const std::vector<ParamLine> flow = {{"flowParams.L1", double(f.L1), false},
+                                         {"flowParams.L2", double(f.L2), false},
+                                         {"flowParams.L3", double(f.L3), false},
+                                         {"flowParams.Re", double(f.Re), false},
+                                         {"flowParams.Ri", double(f.Ri), false},
+                                         {"flowParams.Pr", double(f.Pr), false},
+                                         {"flowParams.EvolveBackground", double(f.EvolveBackground), true}};
+    v.insert(v.end(), flow.begin(), flow.end());
+    return v;
+}
+inline void print_params(std::ostream& out, bool twoAndAHalfFlag) {
+    for (const ParamLine& p : param_lines(twoAndAHalfFlag)) {
+        out << p.name << " ";
+        if (p.integral) out << int(p.value);
+        else out << stratifloat(p.value);
+        out << std::endl;
+    }
+}
+// the text after "<name> " on the first line that starts with name
+inline bool find_param(std::istream& in, const std::string& name, std::string& text) {
+    in.clear();
+    in.seekg(0);
+    for (std::string line; std::getline(in, line);) {
+        if (line.compare(0, name.size(), name) == 0) {
+            text = line.size() > name.size() ? line.substr(name.size() + 1) : std::string();
+            return true;
+        }
+    }
+    std::cerr << "Could not load parameter " << name << std::endl;
+    return false;
+}
+inline void check_param(std::istream& in, const std::string& name, int expected) {
+    std::string text;
+    if (!find_param(in, name, text) || std::stoi(text) != expected)
+        throw std::string("Error: parameter ") + name + " not correct";
+}
+}  // namespace sfacade
+
+// params.dat in the working directory (Parameters.cpp:55-75)
+inline void DumpParameters() {
+    std::ofstream out("params.dat", std::fstream::out);
+    out << std::setprecision(30);
+    sfacade::print_params(out, true);
+}
+// Parameters.cpp:95-111
+inline void PrintParameters() { sfacade::print_params(std::cout, false); }
+// Parameters.cpp:77-93: the grid and EvolveBackground must equal this run's
+// (a mismatch throws std::string), L3 is taken from the file; call
+// StateVector::ResetForParams() afterwards
+inline void LoadParameters(const std::string& file) {
+    std::ifstream in(file, std::fstream::in);
+    const GridParams& g = gridParams;
+    sfacade::check_param(in, "gridParams.N1", g.N1);
+    sfacade::check_param(in, "gridParams.N2", g.N2);
+    sfacade::check_param(in, "gridParams.N3", g.N3);
+    sfacade::check_param(in, "(gridParams.dimensionality == Dimensionality::ThreeDimensional)",
+                         g.dimensionality == Dimensionality::ThreeDimensional);
+    std::string text;
+    if (sfacade::find_param(in, "flowParams.L3", text)) flowParams.L3 = std::stof(text);
+    sfacade::check_param(in, "flowParams.EvolveBackground", flowParams.EvolveBackground);
+}
+
+// root of a x^2 + b x + c (Integration.cpp:3-21): 0 with a message when the
+// roots are complex
+inline stratifloat SolveQuadratic(stratifloat a, stratifloat b, stratifloat c, bool positiveSign = false) {
+    const stratifloat disc = b * b - 4 * a * c;
+    if (disc < 0) {
+        std::cout << "Quadratic gives complex solutions!" << std::endl;
+        std::cout << "a=" << a << ", b=" << b << ", c=" << c << std::endl;
+        return 0;
+    }
+    const stratifloat root = std::sqrt(disc);
+    return (positiveSign ? (-b + root) : (-b - root)) / (2 * a);
+}
 
 // ---------------------------------------------------------------------------
 // Fields
@@ -629,7 +754,8 @@ public:
         return stratifloat(v);
     }
     // IMEXRK.h:183-199 without the image folders
-    void PrepareRunAdjoint() {
+    void PrepareRunAdjoint(const std::string& imageDir = "") {
+        (void)imageDir;
         p.Zero();
         PopulateNodalVariables();
     }
@@ -689,6 +815,24 @@ public:
         UpdateForTimestep();
         return stratifloat(cfl);
     }
+    stratifloat CFLlinear() {  // IMEXRK.h:376-393
+        sync_dt();
+        double cfl = 0;
+        sfacade::check(sf_cfl_linear(c(), &cfl), c());
+        constexpr stratifloat targetCFL = 0.4;
+        deltaT *= targetCFL / stratifloat(cfl);
+        UpdateForTimestep();
+        return stratifloat(cfl);
+    }
+    // IMEXRK.h:164-215: PrepareRun, PrepareRunLinear and PrepareRunAdjoint
+    // populate the nodal fields; the image folders of the reference's plots are
+    // not made (nothing is drawn)
+    void PrepareRun(const std::string& imageDir = "", bool makeDirs = true) {
+        (void)imageDir;
+        (void)makeDirs;
+        PopulateNodalVariables();
+    }
+    void PrepareRunLinear(const std::string& imageDir = "", bool makeDirs = true) { PrepareRun(imageDir, makeDirs); }
     stratifloat KE() const {
         double ke = 0, pe = 0;
         sfacade::check(sf_energy(c(), &ke, &pe), c());
@@ -701,6 +845,12 @@ public:
     }
     // .fields files: nodal U1, U2, U3, B (IMEXRK.h:415-443)
     void SaveFlow(const std::string& filename) {
+        U1.dev(true);
+        U2.dev(true);
+        U3.dev(true);
+        B.dev(true);
+        flush_state();
+        sfacade::check(sf_refresh_nodal(c()), c());
         std::ofstream s(filename, std::ios::out | std::ios::binary);
         U1.Save(s);
         U2.Save(s);
@@ -909,6 +1059,7 @@ public:
         solver().RemoveDivergence(0.0f);
         solver().deltaT = deltaT;
         solver().UpdateForTimestep();
+        solver().PrepareRun("blah", false);
         for (int step = 0; step < steps; ++step) {
             CopyFromSolver(result[step]);
             solver().TimeStep();
@@ -935,12 +1086,13 @@ public:
         CopyFromSolver(result);
     }
 
-    // StateVector.cpp:3-83 without plotting or snapshots
+    // StateVector.cpp:3-83: CFL-adaptive steps to time T that end exactly at T,
+    // a progress line every 100 steps, .fields snapshots in snapshots/ (every
+    // 100 steps and at the end) when snapshot is set, and the time integral of
+    // JoverK returned when calcmixing is set.  screenshot draws nothing here.
     stratifloat FullEvolve(stratifloat T, StateVector& result, bool snapshot = false, bool screenshot = true,
                            bool calcmixing = false) const {
-        (void)snapshot;
         (void)screenshot;
-        (void)calcmixing;
         CopyToSolver();
         solver().SetBackground(InitialU);
         solver().FilterAll();
@@ -948,26 +1100,39 @@ public:
         solver().RemoveDivergence(0.0f);
         stratifloat t = 0.0f;
         int step = 0;
+        solver().PrepareRun("", screenshot);
+        MakeCleanDir("snapshots");
         const int stepinterval = 100;
+        stratifloat mixing = 0;
+        auto save = [&]() {
+            solver().SaveFlow("snapshots/" + std::to_string(step) + "-" + std::to_string(t) + ".fields");
+        };
         while (t + 0.0001 < T) {
             if (step % stepinterval == 0) {
                 solver().CFL();
+                std::cout << step << " " << t << " " << (solver().KE() + solver().PE()) << std::endl;
+                // finish exactly for the last step
                 const stratifloat remaining = T - t;
                 const int remainingSteps = int(remaining / solver().deltaT) + 1;
                 if (remainingSteps < stepinterval) {
                     solver().deltaT = remaining / remainingSteps;
                     solver().UpdateForTimestep();
                 }
+                if (snapshot) save();
             }
             solver().TimeStep();
+            if (calcmixing) mixing += solver().JoverK() * solver().deltaT;
             t += solver().deltaT;
             step++;
+            if (t + 0.0001 >= T && snapshot) save();
         }
         CopyFromSolver(result);
-        return 0;
+        return mixing;
     }
 
-    // StateVector.cpp:107-156
+    // StateVector.cpp:107-156: linearised steps about `about` to time T, dt
+    // from CFLlinear every 100 steps (starting from 0.01) and shortened so
+    // that the last step ends at T
     void LinearEvolve(stratifloat T, const StateVector& about, StateVector& result) const {
         CopyToSolver();
         solver().SetBackground(InitialU);
@@ -976,11 +1141,25 @@ public:
         solver().PopulateNodalVariables();
         solver().RemoveDivergence(0.0f);
         stratifloat t = 0.0f;
+        int step = 0;
+        solver().PrepareRunLinear("", false);
         solver().deltaT = 0.01f;
         solver().UpdateForTimestep();
+        const int stepinterval = 100;
         while (t + 0.0001 < T) {
+            if (step % stepinterval == 0) {
+                solver().CFLlinear();
+                std::cout << step << " " << t << std::endl;
+                const stratifloat remaining = T - t;
+                const int remainingSteps = int(remaining / solver().deltaT) + 1;
+                if (remainingSteps < stepinterval) {
+                    solver().deltaT = remaining / remainingSteps;
+                    solver().UpdateForTimestep();
+                }
+            }
             solver().TimeStepLinear();
             t += solver().deltaT;
+            step++;
         }
         CopyFromSolver(result);
     }

@@ -45,6 +45,7 @@ def test_facade_fixed_evolve_matches_oracle(tmp_path):
     o.populate_nodal()
     o.remove_divergence(0.0)
     o.set_dt(0.01)
+    o.populate_nodal()  # PrepareRun (StateVector.cpp:100)
     init = {n: o.get_modal(n) for n in ("u1", "u3", "b")}
     states = []
     for _ in range(20):
