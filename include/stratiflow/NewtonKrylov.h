@@ -21,7 +21,10 @@
 //     single precision, where a 1e-7 relative step is below FP32 rounding;
 //   * an optional right preconditioner (Precondition / HasPreconditioner)
 //     turns GMRES into its flexible form: the preconditioned directions are
-//     kept beside the Krylov basis and the update is built from them;
+//     kept beside the Krylov basis and the update is built from them; the
+//     trust region then bounds the update itself, |dx| = |Z y| <= Delta, in
+//     the metric of the directions' FP64 Gram matrix (the directions are not
+//     orthonormal), instead of |y|;
 //   * writing ICs/<step>.fields every Newton step is optional (saveICs), and
 //     there is no plotting.
 #pragma once
@@ -131,6 +134,9 @@ public:
                 if (flexible) {
                     if (int(Z.size()) < k) Z.resize(k);
                     Z[k - 1] = Precondition(q[k - 1]);
+                    // Gram matrix of the directions, for the trust region on Z y
+                    if (G.rows < K) G = sfdense::MatrixX(K, K);
+                    for (int j = 0; j < k; ++j) G(j, k - 1) = G(k - 1, j) = sfkrylov::dot(Z[j], Z[k - 1]);
                     q[k] = EvalDerivative(Z[k - 1]);
                 } else {
                     q[k] = EvalDerivative(q[k - 1]);
@@ -149,7 +155,21 @@ public:
             }
             const sfdense::MatrixX subH = H.block(k + 1, k);
             double residual = 0, mu = 0;
-            y = sfdense::hookstep(subH, beta, Delta_, &residual, &mu);
+            sfdense::MatrixX R;
+            if (flexible && Delta_ > 0 && sfdense::cholesky_upper(G, k, R)) {
+                y = sfdense::hookstep_metric(subH, R, beta, Delta_, &residual, &mu);
+            } else {
+                y = sfdense::hookstep(subH, beta, Delta_, &residual, &mu);
+                if (flexible && Delta_ > 0) {
+                    // dependent directions: scale the update back into the region
+                    double zz = 0;
+                    for (int i = 0; i < k; ++i)
+                        for (int j = 0; j < k; ++j) zz += y[i] * G(i, j) * y[j];
+                    const double len = std::sqrt(std::max(zz, 0.0));
+                    if (len > Delta_)
+                        for (double& v : y) v *= Delta_ / len;
+                }
+            }
             if (verbose) {
                 std::cout << "For |z|<Delta, mu=" << mu << std::endl;
                 std::cout << "GMRES STEP " << k << ", RESIDUAL: " << residual << std::endl;
@@ -213,5 +233,6 @@ private:
     int vectorsToReuse = 0;
     std::vector<VectorType> q;
     std::vector<VectorType> Z;  // preconditioned directions (flexible GMRES)
+    sfdense::MatrixX G;         // their FP64 Gram matrix Z^T Z
     sfdense::MatrixX H;         // upper Hessenberg matrix
 };

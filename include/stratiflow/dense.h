@@ -126,7 +126,8 @@ inline void jacobi_svd(const MatrixX& A, MatrixX& U, VectorX& d, MatrixX& V) {
 
 // Least squares min |H y - beta e1| in the singular basis of H, with the
 // trust-region limit |z| <= Delta applied as in NewtonKrylov.h:196-215
-// (z_j = p_j d_j / (d_j^2 + mu), mu raised in steps of 1e-5).  Returns y; the
+// (z_j = p_j d_j / (d_j^2 + mu), mu the first multiple of 1e-5 that meets the
+// bound).  Returns y; the
 // relative residual |H y - beta e1| / beta goes to *residual and the final mu
 // to *mu_out.
 inline VectorX hookstep(const MatrixX& H, double beta, double Delta, double* residual, double* mu_out) {
@@ -138,11 +139,30 @@ inline VectorX hookstep(const MatrixX& H, double beta, double Delta, double* res
     for (int j = 0; j < n; ++j) p[j] = U(0, j) * beta;  // p = U^T (beta e1)
     for (int j = 0; j < n; ++j) z[j] = d[j] > 0 ? p[j] / d[j] : std::numeric_limits<double>::infinity();
     double mu = 0;
-    if (Delta > 0) {
-        while (norm(z) > Delta) {
-            mu += 0.00001;
-            for (int j = 0; j < n; ++j) z[j] = p[j] * d[j] / (d[j] * d[j] + mu);
+    if (Delta > 0 && norm(z) > Delta) {
+        // The reference raises mu in steps of 1e-5 until |z| <= Delta.  |z(mu)|
+        // decreases monotonically in mu, so the first grid point m*1e-5 that
+        // satisfies the bound is found by doubling and bisection over m.
+        auto znorm = [&](double m) {
+            double s2 = 0;
+            for (int j = 0; j < n; ++j) {
+                const double zj = p[j] * d[j] / (d[j] * d[j] + m * 1e-5);
+                s2 += zj * zj;
+            }
+            return std::sqrt(s2);
+        };
+        double lo = 0, hi = 1;  // |z(lo)| > Delta >= |z(hi)| once hi is found
+        while (znorm(hi) > Delta) {
+            lo = hi;
+            hi *= 2;
         }
+        while (hi - lo > 1) {
+            const double mid = std::floor(0.5 * (lo + hi));
+            if (znorm(mid) > Delta) lo = mid;
+            else hi = mid;
+        }
+        mu = hi * 1e-5;
+        for (int j = 0; j < n; ++j) z[j] = p[j] * d[j] / (d[j] * d[j] + mu);
     }
     VectorX y(n, 0.0);
     for (int k = 0; k < n; ++k)
@@ -151,6 +171,51 @@ inline VectorX hookstep(const MatrixX& H, double beta, double Delta, double* res
     r[0] -= beta;
     if (residual) *residual = norm(r) / beta;
     if (mu_out) *mu_out = mu;
+    return y;
+}
+
+// Upper Cholesky factor R (n x n) of a symmetric positive definite G (the
+// top-left n x n block of G is read): G = R^T R.  Returns false when a pivot
+// is not positive (G singular to working precision).
+inline bool cholesky_upper(const MatrixX& G, int n, MatrixX& R) {
+    R = MatrixX(n, n);
+    for (int j = 0; j < n; ++j) {
+        double s = G(j, j);
+        for (int k = 0; k < j; ++k) s -= R(k, j) * R(k, j);
+        if (!(s > 0)) return false;
+        R(j, j) = std::sqrt(s);
+        for (int i = j + 1; i < n; ++i) {
+            double t = G(j, i);
+            for (int k = 0; k < j; ++k) t -= R(k, j) * R(k, i);
+            R(j, i) = t / R(j, j);
+        }
+    }
+    return true;
+}
+
+// Hookstep in a metric: min |H y - beta e1| subject to sqrt(y^T G y) <= Delta,
+// with G = R^T R.  With w = R y the problem is the plain hookstep on H R^-1,
+// so the trust region bounds |Z y| for the directions Z whose Gram matrix is
+// G (the flexible GMRES update dx = Z y).  Same outputs as hookstep.
+inline VectorX hookstep_metric(const MatrixX& H, const MatrixX& R, double beta, double Delta, double* residual,
+                               double* mu_out) {
+    const int m = H.rows, n = H.cols;
+    // HR = H R^-1, column by column (R upper triangular)
+    MatrixX HR(m, n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < m; ++i) {
+            double t = H(i, j);
+            for (int k = 0; k < j; ++k) t -= HR(i, k) * R(k, j);
+            HR(i, j) = t / R(j, j);
+        }
+    VectorX w = hookstep(HR, beta, Delta, residual, mu_out);
+    // y = R^-1 w
+    VectorX y(n, 0.0);
+    for (int i = n - 1; i >= 0; --i) {
+        double t = w[i];
+        for (int k = i + 1; k < n; ++k) t -= R(i, k) * y[k];
+        y[i] = t / R(i, i);
+    }
     return y;
 }
 
