@@ -67,18 +67,23 @@ def _load_state(o, x):
 
 
 def phi_fixed(o, x, dt, nsteps):
-    """nsteps IMEX-RK steps at fixed dt after FixedEvolve's start-up (StateVector.cpp:85-105), p = 0."""
+    """nsteps IMEX-RK steps at fixed dt after FixedEvolve's start-up (StateVector.cpp:85-105,
+    with PrepareRun's populate, IMEXRK.h:164-168), p = 0."""
     _load_state(o, x)
     o.filter_all()
     o.populate_nodal()
     o.remove_divergence(0.0)
     o.set_dt(dt)
+    o.populate_nodal()
     o.step(nsteps)
     return enforce_bcs({n: o.get_modal(n) for n in x})
 
 
-def linear_evolve(o, x, about, T, dt=0.01):
-    """StateVector::LinearEvolve (StateVector.cpp:107-156) with a fixed dt, p = 0."""
+def linear_evolve(o, x, about, T, real=np.float64):
+    """StateVector::LinearEvolve (StateVector.cpp:107-156), p = 0: dt starts at 0.01,
+    CFLlinear (IMEXRK.h:376-393) rescales it to CFL 0.4 every 100 steps, and when
+    fewer than 100 steps remain it is shortened so the last step ends at T.  `real`
+    is the driver's stratifloat, in which t, dt and the step count are kept."""
     tot = {"u1": "u1t", "u2": "u2t", "u3": "u3t", "b": "bt"}
     for n, a in about.items():
         o.set_modal(tot[n], a)
@@ -87,12 +92,23 @@ def linear_evolve(o, x, about, T, dt=0.01):
     o.filter_all()
     o.populate_nodal()
     o.remove_divergence(0.0)
-    o.set_dt(dt)
-    t, n = 0.0, 0
-    while t + 0.0001 < T:
-        t += dt
-        n += 1
-    o.step_linear(n)
+    o.populate_nodal()  # PrepareRunLinear (IMEXRK.h:200-204)
+    T = real(T)
+    dt = real(0.01)
+    o.set_dt(float(dt))
+    t, step = real(0), 0
+    while t + real(0.0001) < T:
+        if step % 100 == 0:
+            dt = real(dt * real(0.4) / real(o.cfl(linear=True)))
+            o.set_dt(float(dt))
+            remaining = real(T - t)
+            steps_left = int(remaining / dt) + 1
+            if steps_left < 100:
+                dt = real(remaining / real(steps_left))
+                o.set_dt(float(dt))
+        o.step_linear(1)
+        t = real(t + dt)
+        step += 1
     return enforce_bcs({k: o.get_modal(k) for k in x})
 
 
