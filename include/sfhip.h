@@ -232,6 +232,16 @@ int sf_cfl(sf_ctx* ctx, double* cfl);
  * for the current dt of the linearised step, from the state set with
  * sf_set_linear_background and 1 as the bound on |U_| */
 int sf_cfl_linear(sf_ctx* ctx, double* cfl);
+/* sf_cfl_linear without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc), since
+ * sf_cfl_linear allocates its nodal scratch field on its first call.  The
+ * scratch (one nodal field and the reduction partials) is a fixed 32 MiB
+ * device array per GPU, held until the process ends: grids whose nodal field
+ * exceeds 32 MiB - 8 KiB (N1*N2*N3*precision bytes) are rejected with
+ * SF_ERR_ARG.  Every call on one GPU shares that array, so calls that overlap
+ * on one GPU (contexts on different streams or threads) race. */
+int sf_cfl_linear_noalloc(sf_ctx* ctx, double* cfl);
 int sf_synchronize(sf_ctx* ctx);
 /* nsteps of sf_step bracketed by HIP events on the context's stream;
  * *ms = elapsed milliseconds */

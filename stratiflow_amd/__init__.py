@@ -59,7 +59,7 @@ EXPORTED = [
     "sf_set_linear_background", "sf_step_adjoint", "sf_jover_k", "sf_field_add_profile", "sf_field_mean_profile",
     "sf_dot_f64", "sf_field_enforce_bcs", "sf_phase_shift", "sf_remove_average", "sf_state_bind", "sf_state_fetch",
     "sf_energy_f64", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
-    "sf_energy", "sf_cfl", "sf_cfl_linear", "sf_refresh_nodal", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
+    "sf_energy", "sf_cfl", "sf_cfl_linear", "sf_cfl_linear_noalloc", "sf_refresh_nodal", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
     "sf_batch_create", "sf_batch_destroy", "sf_batch_last_error", "sf_batch_step", "sf_batch_time_steps",
     "sf_batch_info",
     "sf_net_create", "sf_net_destroy", "sf_net_last_error", "sf_net_apply", "sf_net_add_pair", "sf_net_clear_pairs",
@@ -129,6 +129,7 @@ def lib():
         "sf_energy": [vp, dp, dp],
         "sf_cfl": [vp, dp],
         "sf_cfl_linear": [vp, dp],
+        "sf_cfl_linear_noalloc": [vp, dp],
         "sf_refresh_nodal": [vp],
         "sf_synchronize": [vp],
         "sf_time_steps": [vp, i, dp],

@@ -3,6 +3,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -79,7 +80,9 @@ struct Ctx {
     int N1 = 0, N2 = 0, N3 = 0, M1 = 0, A1 = 0, A2 = 0, dim = 0, N3p = 0;
     bool three = false;
     int zch = 4;   // z levels per x-pass block
+    int device = 0;  // the device current at creation
     double L1 = 0, L2 = 0, L3 = 0, Re = 0, Ri = 0, Pr = 0;
+    double cfl_d[3] = {1, 1, 1};  // L1/N1, L2/N2, z(N3/2+1) - z(N3/2) (IMEXRK.h:359-364)
     double dt = 0.01;
     T h[3] = {0, 0, 0};
     size_t nmodal = 0;  // complex values per modal field
@@ -167,6 +170,7 @@ struct Ctx {
         }
         SF_REQUIRE(!f.evolve_background, SF_ERR_UNSUPPORTED, "EvolveBackground is not supported");
         SF_REQUIRE(!(N1 > 384 && sizeof(T) == 8), SF_ERR_UNSUPPORTED, "FP64 contexts support N1 <= 384");
+        SF_HIP(hipGetDevice(&device));
         M1 = N1 / 2 + 1;
         A1 = N1 / 3;
         A2 = three ? 2 * (N2 / 3) - 1 : 1;
@@ -247,6 +251,9 @@ struct Ctx {
         Pr = f.Pr;
         // vertical operators (Differentiation.cpp:7-111), banded
         const VGrid g = vertical_grid(L3, N3);
+        cfl_d[0] = L1 / N1;
+        cfl_d[1] = L2 / N2;
+        cfl_d[2] = g.z[N3 / 2 + 1] - g.z[N3 / 2];
         std::vector<double> old = zc_host;
         zc_host.assign(size_t(ZC_COUNT) * N3p, 0.0);
         auto Z = [&](int c, int j) -> double& { return zc_host[size_t(c) * N3p + j]; };
@@ -767,8 +774,7 @@ struct Ctx {
         to_nodal(src(0), static_cast<T*>(state[SF_NODAL_U1].d));
         if (three) to_nodal(src(1), static_cast<T*>(state[SF_NODAL_U2].d));
         to_nodal(src(2), static_cast<T*>(state[SF_NODAL_U3].d));
-        const VGrid g = vertical_grid(L3, N3);
-        const double d1 = L1 / N1, d2 = L2 / N2, d3 = g.z[N3 / 2 + 1] - g.z[N3 / 2];
+        const double d1 = cfl_d[0], d2 = cfl_d[1], d3 = cfl_d[2];
         const T* ubar = zc + size_t(ZC_UBAR) * N3p;
         check(launch_maxabs<T>(static_cast<T*>(state[SF_NODAL_U1].d), nnodal, N3, ubar, partial, scal, 0, stream),
               "cfl");
@@ -787,20 +793,25 @@ struct Ctx {
     // IMEXRK::CFLlinear (IMEXRK.h:376-393) for the current dt, without changing
     // it: dt*((1 + max|U1_tot|)/d1 + max|U2_tot|/d2 + max|U3_tot|/d3) over the
     // nodal linearisation state (SetBackground(u1..b), IMEXRK.h:334-352); 1
-    // bounds |U_|.  The nodal totals go through one scratch field in turn.
+    // bounds |U_|.  The nodal totals go through one nodal scratch field in
+    // turn (allocated on the first call), the maxima through the context's
+    // reduction partials.
     T* ntmp = nullptr;
     double cfl_linear() {
         if (!ntmp) ntmp = dalloc<T>(nnodal);
-        const VGrid g = vertical_grid(L3, N3);
-        const double d1 = L1 / N1, d2 = L2 / N2, d3 = g.z[N3 / 2 + 1] - g.z[N3 / 2];
+        return cfl_linear_with(ntmp, partial);
+    }
+    // the same with caller-provided scratch: a nodal field and the partial
+    // sums of the max reductions (at most max_reduction_blocks() doubles)
+    double cfl_linear_with(T* nodal, double* part) {
         double m[3] = {0, 0, 0};
         for (int i = 0; i < 3; ++i) {
             if (i == 1 && !three) continue;
-            to_nodal(mf(SF_U1_TOT + i), ntmp);
-            check(launch_maxabs<T>(ntmp, nnodal, N3, nullptr, partial, scal, 0, stream), "cfl_linear");
+            to_nodal(mf(SF_U1_TOT + i), nodal);
+            check(launch_maxabs<T>(nodal, nnodal, N3, nullptr, part, scal, 0, stream), "cfl_linear");
             read_scalars(1, &m[i]);
         }
-        return ((1.0 + m[0]) / d1 + m[1] / d2 + m[2] / d3) * dt;
+        return ((1.0 + m[0]) / cfl_d[0] + m[1] / cfl_d[1] + m[2] / cfl_d[2]) * dt;
     }
 
     void energy(double* ke, double* pe) {
@@ -1189,6 +1200,41 @@ sf::CtxInfo sf_ctx_info(const sf_ctx* c) {
     return info;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of sf_cfl_linear_noalloc: one array per GPU, held until the process
+// ends, shared by every call on that GPU (overlapping calls on one GPU race).
+// Layout: the partial maxima of the reductions, then one nodal field.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
+constexpr size_t NOALLOC_PARTIAL_BYTES = 8192;  // >= max_reduction_blocks() doubles
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, unsigned char*> g_scratch_ptr;  // device -> address of its scratch
+static std::map<int, std::string> g_scratch_err;
+
+// Resolves the device address of `scratch` on `dev`, which must be current;
+// once per device.  Host code uses only the resolved address, never the
+// array's name (a host-side stand-in).
+static void resolve_scratch(int dev) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr.count(dev) || g_scratch_err.count(dev)) return;
+    unsigned char* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        g_scratch_err[dev] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    else
+        g_scratch_ptr[dev] = ptr;
+}
+
+static unsigned char* scratch_of(int dev) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    auto er = g_scratch_err.find(dev);
+    throw Error(SF_ERR_HIP, er != g_scratch_err.end() ? er->second : "scratch not resolved for this device");
+}
+
 extern "C" {
 
 int sf_ctx_create(const sf_grid* grid, const sf_flow* flow, int precision, sf_ctx** out) {
@@ -1219,6 +1265,8 @@ int sf_ctx_create(const sf_grid* grid, const sf_flow* flow, int precision, sf_ct
         }
     });
     if (rc != SF_OK) return rc;
+    // the context was created on the current device
+    with_ctx(c.get(), [&](auto& C) { resolve_scratch(C.device); });
     *out = c.release();
     return SF_OK;
 }
@@ -1679,6 +1727,27 @@ int sf_cfl(sf_ctx* ctx, double* cfl) {
     SF_NULLCHK(ctx);
     SF_NULLCHK(cfl);
     return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { *cfl = C.cfl(); }); });
+}
+
+int sf_cfl_linear_noalloc(sf_ctx* ctx, double* cfl) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(cfl);
+    return guarded(ctx, [&]() {
+        with_ctx(ctx, [&](auto& C) {
+            using T = typename std::remove_reference<decltype(C.h[0])>::type;
+            const size_t need = C.nnodal * sizeof(T);
+            SF_REQUIRE(need <= NOALLOC_SCRATCH_BYTES - NOALLOC_PARTIAL_BYTES, SF_ERR_ARG,
+                       "sf_cfl_linear_noalloc: a nodal field of " + std::to_string(need) +
+                           " bytes exceeds the scratch of " +
+                           std::to_string(NOALLOC_SCRATCH_BYTES - NOALLOC_PARTIAL_BYTES) +
+                           " bytes (use sf_cfl_linear)");
+            SF_REQUIRE(size_t(max_reduction_blocks()) * sizeof(double) <= NOALLOC_PARTIAL_BYTES, SF_ERR_STATE,
+                       "sf_cfl_linear_noalloc: reduction partials exceed their scratch");
+            unsigned char* base = scratch_of(C.device);
+            *cfl = C.cfl_linear_with(reinterpret_cast<T*>(base + NOALLOC_PARTIAL_BYTES),
+                                     reinterpret_cast<double*>(base));
+        });
+    });
 }
 
 int sf_refresh_nodal(sf_ctx* ctx) {

@@ -167,6 +167,8 @@ int xpass_levels(int N1, int N2, int N3, bool threeD, int precision);
 // twiddle table of an n-point LDS FFT (the layout its second pass reads)
 void fft_twiddles(int n, double* re, double* im);
 int max_rows_per_lane();
+// blocks (and so partial sums) of the two-pass FP64 reductions (dot, max-abs)
+int max_reduction_blocks();
 // modes per block of the modal kernels for M rows per lane
 int modal_modes_per_block(int M, int precision);
 

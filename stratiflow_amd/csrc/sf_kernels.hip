@@ -1263,6 +1263,8 @@ hipError_t launch_dot(const cx<T>* a, const cx<T>* b, int A1, int A2, int N3, co
     return hipGetLastError();
 }
 
+int max_reduction_blocks() { return DOT_BLOCKS; }
+
 // max |f + add[z]| over a nodal field (NodalField::Max, Field.h:771-786)
 template <typename T>
 __global__ __launch_bounds__(BS) void k_maxabs_partial(const T* __restrict__ f, size_t n, int N3,
