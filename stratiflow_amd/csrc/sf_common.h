@@ -102,6 +102,47 @@ __host__ __device__ __forceinline__ V czero() {
     return r;
 }
 
+// ---- raw buffer access ------------------------------------------------------
+// Loads and stores through a buffer resource (base and size in SGPRs) with a
+// 32-bit byte offset per lane: no 64-bit address arithmetic per access, and an
+// offset at or beyond the size reads zero and drops a store, so masked
+// accesses need no branches.  Every array addressed this way is below 2 GiB.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int BUF_OOB = int(0x80000000u);  // an offset outside every buffer
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, int(unsigned(bytes)), 0x00020000);
+}
+template <typename V>
+__device__ __forceinline__ V bld(rsrc_t r, int voff, int soff = 0) {
+    if constexpr (sizeof(V) == 4) {
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    } else if constexpr (sizeof(V) == 8) {
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    } else {
+        static_assert(sizeof(V) == 16, "buffer access of 4, 8 or 16 bytes");
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    }
+}
+template <typename V>
+__device__ __forceinline__ void bst(V v, rsrc_t r, int voff, int soff = 0) {
+    if constexpr (sizeof(V) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+    } else if constexpr (sizeof(V) == 8) {
+        using U2 = unsigned __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, voff, soff, 0);
+    } else {
+        static_assert(sizeof(V) == 16, "buffer access of 4, 8 or 16 bytes");
+        using U4 = unsigned __attribute__((ext_vector_type(4)));
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, v), r, voff, soff, 0);
+    }
+}
+
+// 1/x: the hardware reciprocal (1 ulp) in FP32, a correctly rounded division
+// in FP64
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 // Per-z coefficient table rows (filled by the context set-up in sf_api.hip).
 enum ZCoef {
     ZC_DZZN_LO = 0,

@@ -55,9 +55,10 @@ __device__ __forceinline__ void nb_next(const cx<T> (&x)[M], cx<T> (&xp)[M], int
 // kind: 0 velocity Neumann, 1 velocity Dirichlet, 2 buoyancy Neumann,
 // 3 pressure Laplacian.  Matrix rows as built in IMEXRK.h:55-78, 502-545
 // with Neumannify/Dirichlify (Differentiation.h:17-59).
-template <typename T, int M>
+// zcoef(c, t): coefficient row c at the lane's row t (zero beyond N3).
+template <typename T, int M, typename ZC>
 __device__ __forceinline__ void build_rows(int kind, const ModalArgs<T>& A, const int (&row)[M], T kk, T kl,
-                                           bool zero_mode, T coef, T (&a)[M], T (&b)[M], T (&c)[M]) {
+                                           bool zero_mode, T coef, ZC zcoef, T (&a)[M], T (&b)[M], T (&c)[M]) {
     const int N = A.N3;
     const bool dir = (kind == 1);
 #pragma unroll
@@ -69,9 +70,9 @@ __device__ __forceinline__ void build_rows(int kind, const ModalArgs<T>& A, cons
             c[t] = 0;
             continue;
         }
-        const T lo = A.zc[(dir ? ZC_DZZD_LO : ZC_DZZN_LO) * A.N3p + j];
-        const T di = A.zc[(dir ? ZC_DZZD_DI : ZC_DZZN_DI) * A.N3p + j];
-        const T up = A.zc[(dir ? ZC_DZZD_UP : ZC_DZZN_UP) * A.N3p + j];
+        const T lo = zcoef(dir ? ZC_DZZD_LO : ZC_DZZN_LO, t);
+        const T di = zcoef(dir ? ZC_DZZD_DI : ZC_DZZN_DI, t);
+        const T up = zcoef(dir ? ZC_DZZD_UP : ZC_DZZN_UP, t);
         const T ldi = (di + kk) + kl;
         T aa, bb, cc;
         if (kind == 3) {
@@ -147,28 +148,38 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
     (void)active;
 
     int row[M];
-    bool ok[M];
 #pragma unroll
-    for (int t = 0; t < M; ++t) {
-        row[t] = lane * M + t;
-        ok[t] = row[t] < N;
-    }
-    auto coefz = [&](int c, int t) -> T { return ok[t] ? A.zc[c * A.N3p + row[t]] : T(0); };
+    for (int t = 0; t < M; ++t) row[t] = lane * M + t;
+    // coefficient table: row c of the lane's row t through the scalar offset
+    // c * N3p; rows at or beyond N3 read zero
+    const rsrc_t zr = make_rsrc(A.zc, size_t(ZC_COUNT) * A.N3p * sizeof(T));
+    int zoff[M];
+#pragma unroll
+    for (int t = 0; t < M; ++t) zoff[t] = row[t] < N ? row[t] * int(sizeof(T)) : BUF_OOB;
+    auto coefz = [&](int c, int t) -> T { return bld<T>(zr, zoff[t], c * A.N3p * int(sizeof(T))); };
     // tile element owned by this thread in the cooperative (coalesced) phase:
-    // rows z = tid/MB + t*64 for t < M (N3 <= 64*M), mode w = tid % MB
+    // rows z = tid/MB + t*64 for t < M (N3 <= 64*M), mode w = tid % MB.  Every
+    // modal field has the same layout, so one table of byte offsets serves all
+    // of them; elements outside the field (z >= N3, j1 >= A1) get an
+    // out-of-range offset and read zero / are not written.
     const int cw = int(threadIdx.x) % MB;
     const int cz = int(threadIdx.x) / MB;
     const int cj1 = j10 + cw;
-    // issue: global loads of this thread's tile elements into g (left in
-    // flight); finish: through the LDS tile into the wave's column x.  Several
-    // issues before their finishes overlap the memory latency of those fields.
+    const size_t fbytes = size_t(A1) * size_t(A.A2) * size_t(N) * sizeof(V);
+    int foff[M];
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+        const int z = cz + t * 64;
+        foff[t] = (z < N && cj1 < A1) ? int(sidx(ja, z, cj1, N, A1) * sizeof(V)) : BUF_OOB;
+    }
+    // issue: loads of this thread's tile elements into g (left in flight);
+    // finish: through the LDS tile into the wave's column x.  Several issues
+    // before their finishes overlap the memory latency of those fields.
     auto issue = [&](const V* f, V (&g)[M]) {
         if (f == nullptr) return;  // block-uniform
+        const rsrc_t r = make_rsrc(f, fbytes);
 #pragma unroll
-        for (int t = 0; t < M; ++t) {
-            const int z = cz + t * 64;
-            g[t] = (z < N && cj1 < A1) ? f[sidx(ja, z, cj1, N, A1)] : czero<V>();
-        }
+        for (int t = 0; t < M; ++t) g[t] = bld<V>(r, foff[t]);
     };
     auto finish = [&](const V* f, const V (&g)[M], V (&x)[M]) {
         if (f == nullptr) {  // block-uniform
@@ -177,27 +188,23 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
             return;
         }
         sync_lds();  // the previous tile has been read by every wave
+        // rows z >= N3 receive the zeros their loads returned, so the padding
+        // rows of every column are zero
 #pragma unroll
-        for (int t = 0; t < M; ++t) {
-            const int z = cz + t * 64;
-            if (z < N) tile[z * TP + cw] = g[t];
-        }
+        for (int t = 0; t < M; ++t) tile[(cz + t * 64) * TP + cw] = g[t];
         sync_lds();
 #pragma unroll
-        for (int t = 0; t < M; ++t) x[t] = ok[t] ? tile[row[t] * TP + wave] : czero<V>();
+        for (int t = 0; t < M; ++t) x[t] = tile[row[t] * TP + wave];
     };
     auto store = [&](V* f, const V (&x)[M]) {
         if (f == nullptr) return;
+        const rsrc_t r = make_rsrc(f, fbytes);
         sync_lds();
 #pragma unroll
-        for (int t = 0; t < M; ++t)
-            if (ok[t]) tile[row[t] * TP + wave] = x[t];
+        for (int t = 0; t < M; ++t) tile[row[t] * TP + wave] = x[t];
         sync_lds();
 #pragma unroll
-        for (int t = 0; t < M; ++t) {
-            const int z = cz + t * 64;
-            if (z < N && cj1 < A1) f[sidx(ja, z, cj1, N, A1)] = tile[z * TP + cw];
-        }
+        for (int t = 0; t < M; ++t) bst<V>(tile[(cz + t * 64) * TP + cw], r, foff[t]);
     };
     // ZeroEnds, Field.h:475-486
     auto zero_ends = [&](V (&x)[M], bool dir) {
@@ -286,7 +293,7 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         TriFactor<T, M> F;
         T ta[M], tb[M], tc[M];
         // u1 and u2 share the velocity Neumann matrix
-        build_rows<T, M>(0, A, row, kk, kl, zero_mode, A.cv, ta, tb, tc);
+        build_rows<T, M>(0, A, row, kk, kl, zero_mode, A.cv, coefz, ta, tb, tc);
         tri_factor<T, M>(F, ta, tb, tc, lane);
 #pragma unroll
         for (int t = 0; t < M; ++t) gp[t] = cik(kx, pp[t]);
@@ -312,7 +319,7 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
 #pragma unroll
             for (int t = 0; t < M; ++t) gp[t] = cscale(coefz(ZC_DZND, t), csub(pp[t], pm[t]));
         }
-        build_rows<T, M>(1, A, row, kk, kl, zero_mode, A.cv, ta, tb, tc);
+        build_rows<T, M>(1, A, row, kk, kl, zero_mode, A.cv, coefz, ta, tb, tc);
         tri_factor<T, M>(F, ta, tb, tc, lane);
         assemble(R, u3, A.ro3, g2, rn3, gp, true, A.cv);
         issue(A.roB, g3);
@@ -322,7 +329,7 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         // b: buoyancy Neumann matrix, no pressure term
 #pragma unroll
         for (int t = 0; t < M; ++t) gp[t] = czero<V>();
-        build_rows<T, M>(2, A, row, kk, kl, zero_mode, A.cb, ta, tb, tc);
+        build_rows<T, M>(2, A, row, kk, kl, zero_mode, A.cb, coefz, ta, tb, tc);
         tri_factor<T, M>(F, ta, tb, tc, lane);
         assemble(R, bb, A.roB, g3, rnB, gp, false, A.cb);
         tri_apply<T, M>(F, R, lane);
@@ -351,7 +358,7 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
             TriFactor<double, M> F;
             double ta[M], tb[M], tc[M];
             T fa[M], fb[M], fc[M];
-            build_rows<T, M>(3, A, row, kk, kl, zero_mode, T(0), fa, fb, fc);
+            build_rows<T, M>(3, A, row, kk, kl, zero_mode, T(0), coefz, fa, fb, fc);
 #pragma unroll
             for (int t = 0; t < M; ++t) {
                 ta[t] = fa[t];
@@ -379,7 +386,7 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         } else {
             TriFactor<T, M> F;
             T ta[M], tb[M], tc[M];
-            build_rows<T, M>(3, A, row, kk, kl, zero_mode, T(0), ta, tb, tc);
+            build_rows<T, M>(3, A, row, kk, kl, zero_mode, T(0), coefz, ta, tb, tc);
             tri_factor<T, M>(F, ta, tb, tc, lane);
             tri_apply<T, M>(F, dv, lane);  // dv now holds q
             V qm[M];
@@ -1426,7 +1433,8 @@ __global__ __launch_bounds__(64) void k_diffprec(DiffPrecArgs<T> P) {
         A.N3p = P.N3p;
         A.zc = P.zc;
         T a[M], b[M], c[M];
-        build_rows<T, M>(dir ? 1 : 0, A, row, kk, kl, false, P.tnu / P.delta, a, b, c);
+        auto zcoef = [&](int cr, int t) -> T { return row[t] < N ? P.zc[cr * P.N3p + row[t]] : T(0); };
+        build_rows<T, M>(dir ? 1 : 0, A, row, kk, kl, false, P.tnu / P.delta, zcoef, a, b, c);
         TriFactor<T, M> F;
         tri_factor<T, M>(F, a, b, c, lane);
         tri_apply<T, M>(F, y, lane);

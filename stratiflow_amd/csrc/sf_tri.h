@@ -14,7 +14,8 @@
 // Matrix-only quantities (local factors, spikes v and w, PCR multipliers) are
 // computed once per matrix and reused for every right-hand side that shares
 // it.  Coefficients are real; right-hand sides are complex (real and
-// imaginary parts see identical arithmetic, as in Field.h:520-531).
+// imaginary parts see identical arithmetic, as in Field.h:520-531).  In FP32
+// the pivots are inverted with the hardware reciprocal (1 ulp).
 // Padding rows beyond N3 must be identity rows with zero right-hand side.
 #pragma once
 #include "sf_common.h"
@@ -56,12 +57,12 @@ __device__ __forceinline__ void tri_factor(TriFactor<T, M>& F, const T (&a)[M], 
     } else {
         // interior rows 0..M-2
         T den = b[0];
-        F.rden[0] = T(1) / den;
+        F.rden[0] = recip(den);
 #pragma unroll
         for (int t = 0; t < M - 2; ++t) {
             F.cc[t] = c[t] * F.rden[t];
             den = b[t + 1] - a[t + 1] * F.cc[t];
-            F.rden[t + 1] = T(1) / den;
+            F.rden[t + 1] = recip(den);
         }
         // spike v: A_int^{-1} (a_0 e_0)
         {
@@ -109,8 +110,8 @@ __device__ __forceinline__ void tri_factor(TriFactor<T, M>& F, const T (&a)[M], 
             Bp = 1;
             Cp = 0;
         }
-        const T al = -As / Bm;
-        const T ga = -Cs / Bp;
+        const T al = -As * recip(Bm);
+        const T ga = -Cs * recip(Bp);
         F.alpha[lev] = al;
         F.gamma[lev] = ga;
         const T nA = al * Am;
@@ -119,7 +120,7 @@ __device__ __forceinline__ void tri_factor(TriFactor<T, M>& F, const T (&a)[M], 
         As = nA;
         Cs = nC;
     }
-    F.rB = T(1) / Bs;
+    F.rB = recip(Bs);
 }
 
 // Solve in place: d holds the lane's M right-hand-side rows on entry and the
