@@ -12,9 +12,11 @@ kernel's algorithmic bytes per launch over its average launch time;
 ``step_roofline`` applies the per-step byte model to the whole step;
 ``cpu_baseline`` times the CPU oracle (a C++ restatement of the reference,
 OpenMP) on the host cores for a bounded sample of the same workload.
-``newton_gmres`` (after the timed region, from tests/cpp/precond_demo when it
-is built) compares plain and preconditioned Newton-GMRES solves on the 8
-held-out 64x1x64 problems: operator applications, wall time, problems solved.
+``newton_gmres`` (after the timed region; tests/cpp/newton_bench.cpp, compiled
+here when its binary is missing) runs Newton-GMRES toward relative residual
+1e-8 on held-out 256x1x256 problems with plain GMRES, the analytic
+preconditioner and the learned network on it: Newton steps, operator
+applications, wall time to each residual decade, final residuals.
 """
 import subprocess
 import argparse
@@ -158,34 +160,94 @@ def ensemble(sf, ic, nruns=64, steps=500, seed=20261202):
     return out
 
 
-def newton_gmres(timeout=240):
-    """One GMRES solve per problem of F(x) = Phi_2(x) - x on the 8 held-out 64x1x64
-    problems (Re = 100, Krylov limit 60, tolerance 2e-2), plain and right-
-    preconditioned by the advected diffusion preconditioner (delta = 0.3);
-    None when the demo program has not been built."""
-    binary = os.path.join(HERE, "tests", "cpp", "_build", "precond_demo")
-    if not os.path.exists(binary):
-        return None
-    r = subprocess.run([binary, "8", "61", "2e-2", "100", "0.3", "100", "1"], capture_output=True, text=True,
-                       timeout=timeout)
+NEWTON_ARGS = ["ntrain=2", "ntest=2", "newton=10", "batches=3000"]
+
+
+def newton_binary():
+    """tests/cpp/newton_bench.cpp built against the in-tree library (FP64
+    stepping); built here when missing.  Returns (path, error)."""
+    exe = os.path.join(HERE, "tests", "cpp", "_build", "newton_bench_f64")
+    if os.path.exists(exe):
+        return exe, None
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    lib = os.path.join(HERE, "stratiflow_amd")
+    try:
+        r = subprocess.run(["g++", "-std=c++14", "-O2", "-DUSE_DOUBLE", "-I" + os.path.join(HERE, "include"), "-o",
+                            exe, os.path.join(HERE, "tests", "cpp", "newton_bench.cpp"), "-L" + lib, "-lsfhip",
+                            "-Wl,-rpath," + lib], capture_output=True, text=True, timeout=300)
+    except (OSError, subprocess.TimeoutExpired) as e:
+        return None, "building newton_bench failed: %s" % e
+    if r.returncode != 0:
+        return None, "building newton_bench failed: " + r.stderr[-400:]
+    return exe, None
+
+
+def newton_gmres(timeout=480):
+    """Newton-GMRES toward |F| <= 1e-8 |F(x0)| on held-out problems of the
+    learned-preconditioner class (256x1x256, Re in [1000, 3000], Ri in
+    [0.08, 0.2], F(x) = Phi_2(x) - x, FP64 stepping, Krylov dimension 60, GMRES
+    tolerance 1e-2, at most 10 Newton steps): plain GMRES, the analytic
+    advected-diffusion preconditioner, and the learned network on that base
+    (trained on the GPU on the operator applications of two plain training
+    runs), judged by plain GMRES."""
+    exe, err = newton_binary()
+    if err:
+        return {"error": err}
+    try:
+        r = subprocess.run([exe] + NEWTON_ARGS, capture_output=True, text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": "newton_bench exceeded %d s" % timeout}
+    except OSError as e:
+        return {"error": "newton_bench could not run: %s" % e}
     if r.returncode != 0:
         return {"error": (r.stdout + r.stderr)[-400:]}
-    rows = [line.split() for line in r.stdout.splitlines() if line.startswith("test ")]
-    tol = 2e-2
-    plain_s = sum(float(t[3]) for t in rows)
-    pre_s = sum(float(t[5]) for t in rows)
-    both = [t for t in rows if float(t[6]) < tol and float(t[7]) < tol]
+    methods = {}
+    loss = None
+    for line in r.stdout.splitlines():
+        t = line.split()
+        if not t:
+            continue
+        if t[0] == "loss":
+            loss = {"first": float(t[1]), "last": float(t[2]), "samples": int(t[3]), "train_s": float(t[4])}
+        elif t[0] == "test":
+            m = methods.setdefault(t[4], {"newton_steps": 0, "matvecs": 0, "seconds": 0.0, "final_rel_residual": [],
+                                          "fallbacks": 0, "time_to": {}})
+            m["newton_steps"] += int(t[5])
+            m["matvecs"] += int(t[6])
+            m["seconds"] += float(t[7])
+            m["final_rel_residual"].append(float(t[8]))
+            m["fallbacks"] += int(t[9])
+        elif t[0] == "hist":
+            m = methods[t[2]]
+            for e in range(2, 9):
+                thr = 10.0 ** -e
+                hit = next((float(p.split("@")[1]) for p in t[3:] if float(p.split("@")[0]) <= thr), None)
+                key = "1e-%d" % e
+                if hit is None:
+                    m["time_to"][key] = None
+                elif m["time_to"].get(key, 0.0) is not None:
+                    m["time_to"][key] = m["time_to"].get(key, 0.0) + hit
+    for m in methods.values():
+        m["reached_1e-8"] = m["time_to"].get("1e-8") is not None
+    # the smallest residual decade that plain and learned both reach on every problem
+    ratio, level = None, None
+    p, l = methods.get("plain"), methods.get("learned")
+    if p and l:
+        for e in range(8, 1, -1):
+            key = "1e-%d" % e
+            if p["time_to"].get(key) and l["time_to"].get(key):
+                ratio, level = p["time_to"][key] / l["time_to"][key], key
+                break
     return {
-        "workload": "one Newton-GMRES solve per problem, 8 held-out problems, 64x1x64 2D, Re=100, T=2, K=60, "
-                    "tol 2e-2; preconditioner (T L_k - 0.3)^-1 - 1/2, L_k = nu(Dzz-k^2) - i kx U(z)",
-        "plain_matvecs": sum(int(t[2]) for t in rows),
-        "precond_matvecs": sum(int(t[4]) for t in rows),
-        "plain_s": plain_s,
-        "precond_s": pre_s,
-        "plain_solved": sum(float(t[6]) < tol for t in rows),
-        "precond_solved": sum(float(t[7]) < tol for t in rows),
-        "time_ratio_where_both_solve": (sum(float(t[3]) for t in both) / sum(float(t[5]) for t in both))
-        if both else None,
+        "workload": "Newton-GMRES to relative residual 1e-8 (at most 10 Newton steps), 2 held-out problems, "
+                    "256x1x256 2D, Re~U[1000,3000], Ri~U[0.08,0.2], F(x)=Phi_2(x)-x, FP64 stepping, K=60, "
+                    "GMRES tol 1e-2; learned = 4 x Conv1D(64 ch, k=5, dilations 1,2,4,8) BF16/MFMA network "
+                    "correcting the analytic advected-diffusion preconditioner (delta 0.3), trained on the "
+                    "GPU on the operator applications of 2 plain training runs",
+        "methods": methods,
+        "training_loss": loss,
+        "time_ratio_plain_over_learned": ratio,
+        "at_residual": level,
     }
 
 

@@ -30,6 +30,7 @@
 #pragma once
 
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <iostream>
 #include <string>
@@ -71,6 +72,8 @@ public:
         VectorType dx;
         int step = 0;
         residualHistory.clear();
+        residualSeconds.clear();
+        const auto start = std::chrono::steady_clock::now();
         while (true) {
             step++;
             if (saveICs) x.SaveToFile("ICs/" + std::to_string(step));
@@ -79,6 +82,8 @@ public:
             linearAboutEnd = rhs;
             const double residual = sfkrylov::norm(rhs);
             residualHistory.push_back(residual);
+            residualSeconds.push_back(
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count());
             if (verbose) std::cout << "NEWTON STEP " << step << ", RESIDUAL: " << residual << std::endl;
             newtonSteps = step;
             if (residual < targetResidual || step > maxNewtonSteps) return;
@@ -111,6 +116,7 @@ public:
     int lastKrylovDimension = 0;
     double lastGmresResidual = 0;
     std::vector<double> residualHistory;
+    std::vector<double> residualSeconds;  // wall time since Run began at each residual
     std::vector<double> gmresResidualHistory;  // every GMRES step of the last solve
 
     // solves A x = rhs with A = -dF/dx (NewtonKrylov.h:134-235); public so

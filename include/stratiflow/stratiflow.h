@@ -571,22 +571,24 @@ inline void ModalField::ExciteLowWavenumbers(stratifloat L) {
     const std::vector<double>& zz = (bc_ == BoundaryCondition::Dirichlet) ? z : zf;
     const GridParams& g = gridParams;
     const int M2 = g.ThirdDimension() ? g.N2 / 6 : g.N2 / 2;
-    std::uniform_real_distribution<stratifloat> dist(-1.0, 1.0);
+    // the draws and the profile are evaluated in double whatever stratifloat
+    // is, so a seeded state is the same (to rounding) in FP32 and FP64 builds
+    std::uniform_real_distribution<double> dist(-1.0, 1.0);
     std::mt19937& gen = sfacade::rng();
     for (int j1 = 0; j1 < g.N1 / 6; j1++) {
         for (int j2 = -M2; j2 <= M2; j2++) {
-            complex coeff[4];
+            std::complex<double> coeff[4];
             for (int h = 0; h < 4; h++) {
-                const stratifloat re = dist(gen);
-                const stratifloat im = dist(gen);
-                coeff[h] = stratifloat(std::pow(double(j1 * j1 + j2 * j2 + h * h + 1), -5.0 / 3.0)) * complex(re, im);
+                const double re = dist(gen);
+                const double im = dist(gen);
+                coeff[h] = std::pow(double(j1 * j1 + j2 * j2 + h * h + 1), -5.0 / 3.0) * std::complex<double>(re, im);
             }
             const int actualj2 = j2 < 0 ? j2 + g.N2 : j2;
             for (int j3 = 0; j3 < g.N3; j3++) {
-                complex v = 0;
-                const stratifloat zj = stratifloat(zz[j3]);
-                for (int h = 0; h < 4; h++) v += coeff[h] * std::exp(-zj * zj * 2) * Hermite(h, zj * 2);
-                (*this)(j1, actualj2, j3) = v;
+                std::complex<double> v = 0;
+                const double zj = zz[j3];
+                for (int h = 0; h < 4; h++) v += coeff[h] * std::exp(-zj * zj * 2) * double(Hermite(h, stratifloat(zj * 2)));
+                (*this)(j1, actualj2, j3) = complex(v);
             }
         }
     }

@@ -1,6 +1,7 @@
 """Parity at the throughput configuration, 384 x 192 x 384 (3D), through the C ABI.
 
-* one full TimeStep of the seeded workload (Re=4000, Ri=0.16, dt=0.005) against
+* one full TimeStep of the seeded workload (Re=4000, Ri=0.16, dt=0.005, tanh U and
+  tanh background buoyancy, as bench.py times it) against
   the FP64 oracle on the same inputs: FP64 to 1e-10, FP32 to 2e-5 relative;
 * size-independent properties: ToModal(ToNodal(u)) = u on filtered spectra,
   and the projection leaves a discrete divergence at rounding level.
@@ -22,7 +23,10 @@ def _pair(prec):
     import oracle
     oracle.set_threads(16)
     N1, N2, N3, dim = C2
-    return make_pair(N1, N2, N3, dim, Re=4000.0, Ri=0.16, dt=0.005, seed=20261202, gpu_prec=prec, amplitude=1e-3)
+    from stratiflow_amd import ic
+    # the throughput workload: tanh U and tanh background buoyancy
+    return make_pair(N1, N2, N3, dim, Re=4000.0, Ri=0.16, dt=0.005, seed=20261202, gpu_prec=prec, amplitude=1e-3,
+                     stratification=ic.tanh_stratification)
 
 
 def test_full_size_step_fp64_matches_oracle():
@@ -53,7 +57,7 @@ def test_full_size_step_fp32_within_fp32_rounding():
     o32 = oracle.Oracle(N1, N2, N3, dim, 2 * np.pi / 0.44, np.pi / 0.44, 10.0, 4000.0, 0.16, 1.0, precision=4)
     from stratiflow_amd import ic
     z, zf, _, _ = o32.grid()
-    o32.set_background(ic.tanh_background(zf))
+    o32.set_background(ic.tanh_background(zf), ic.tanh_stratification(zf))
     for n in ("u1", "u2", "u3", "b", "p"):
         o32.set_modal(n, o.get_modal(n))
     for n in ("U1", "U2", "U3", "B"):

@@ -115,14 +115,16 @@ def test_steps_fp32_energy(shape):
 
 
 def test_c1_energy_series_1000_steps():
-    """The 2D workload 128x1x128 (Re=1000, Ri=0.1, Pr=1, tanh U, dt=0.01, seeded
-    perturbation of amplitude 1e-3) for 1000 steps: the FP32 GPU KE/PE series
+    """The 2D workload 128x1x128 (Re=1000, Ri=0.1, Pr=1, tanh U and tanh background
+    buoyancy, dt=0.01, seeded perturbation of amplitude 1e-3) for 1000 steps: the FP32 GPU KE/PE series
     every 100 steps against the FP32 and FP64 oracles, to 1e-4 relative."""
     import oracle
     oracle.set_threads(16)
     N1, N2, N3, dim = 128, 1, 128, 1
-    s, o32 = make_pair(N1, N2, N3, dim, gpu_prec=4, oracle_prec=4, amplitude=1e-3, seed=20261202)
-    _, o64 = make_pair(N1, N2, N3, dim, gpu_prec=4, oracle_prec=8, amplitude=1e-3, seed=20261202)
+    from stratiflow_amd import ic
+    st = ic.tanh_stratification  # background buoyancy B = tanh z
+    s, o32 = make_pair(N1, N2, N3, dim, gpu_prec=4, oracle_prec=4, amplitude=1e-3, seed=20261202, stratification=st)
+    _, o64 = make_pair(N1, N2, N3, dim, gpu_prec=4, oracle_prec=8, amplitude=1e-3, seed=20261202, stratification=st)
     worst = 0.0
     for chunk in range(10):
         s.TimeStep(100)
