@@ -73,7 +73,9 @@ typedef struct sf_grid {
     int dimensionality;
 } sf_grid;
 
-/* FlowParams, Parameters.h:21-30 */
+/* FlowParams, Parameters.h:21-30.  evolve_background != 0: every stage's
+ * Crank-Nicolson step also diffuses the background U_ (IMEXRK.cpp:100-104),
+ * in the linearised and adjoint steps as well; sf_get_background reads it. */
 typedef struct sf_flow {
     double L1, L2, L3, Re, Ri, Pr;
     int evolve_background;
@@ -102,6 +104,9 @@ int sf_set_flow(sf_ctx* ctx, const sf_flow* flow);
  * values of U_.  dBdz (N3 values at Neumann points, may be NULL = 1) is the
  * background stratification gradient that IMEXRK.cpp:145-147 fixes at 1. */
 int sf_set_background(sf_ctx* ctx, const double* U, const double* dBdz);
+/* the current U_ (IMEXRK.h:609), N3 values: what sf_set_background set, as
+ * the steps have evolved it when evolve_background is set */
+int sf_get_background(sf_ctx* ctx, double* U);
 /* VerticalPoints and VerticalPointsFractional (Field.cpp:3-31): N3 values
  * each (z[0] is extrapolated; the reference leaves it uninitialised) */
 int sf_vertical_points(const sf_ctx* ctx, double* z, double* zf);

@@ -726,10 +726,20 @@ public:
     void TimeStep() {                                                          // IMEXRK.cpp:8-24
         sync_dt();
         sfacade::check(sf_step(c(), 1), c());
+        sync_background();
     }
     void TimeStepLinear() {  // IMEXRK.cpp:26-41
         sync_dt();
         sfacade::check(sf_step_linear(c(), 1), c());
+        sync_background();
+    }
+    // U_ as the steps left it when flowParams.EvolveBackground is set
+    // (IMEXRK.cpp:100-104)
+    void sync_background() {
+        if (!flowParams.EvolveBackground) return;
+        std::vector<double> u(gridParams.N3);
+        sfacade::check(sf_get_background(c(), u.data()), c());
+        for (int k = 0; k < gridParams.N3; ++k) U_.Get()[k] = stratifloat(u[k]);
     }
     // IMEXRK.h:86-126: one backward step of the adjoint state in u1..b with the
     // direct state at the end (Below) and start (Above) of the step
@@ -743,6 +753,7 @@ public:
         const sf_field* above[4] = {u1Above.dev(false), three ? u2Above.dev(false) : nullptr, u3Above.dev(false),
                                     bAbove.dev(false)};
         sfacade::check(sf_step_adjoint(c(), below, above), c());
+        sync_background();
         U1.dev(true);
         U2.dev(true);
         U3.dev(true);

@@ -75,6 +75,8 @@ def lib():
         L.sfo_jover_k.restype = d
         L.sfo_cfl.argtypes = [vp, i]
         L.sfo_cfl.restype = d
+        L.sfo_set_evolve_background.argtypes = [vp, i]
+        L.sfo_get_background.argtypes = [vp, dp]
         L.sfo_adjoint.argtypes = [vp, i, ctypes.POINTER(dp), ctypes.POINTER(dp), d]
         _lib = L
     return _lib
@@ -173,6 +175,17 @@ class Oracle:
     def jover_k(self):
         """IMEXRK::JoverK (IMEXRK.h:445-459) from the nodal B and the modal u3."""
         return lib().sfo_jover_k(self.h)
+
+    def set_evolve_background(self, on=True):
+        """FlowParams::EvolveBackground: U_ takes a Crank-Nicolson diffusion solve per
+        stage (IMEXRK.cpp:100-104)."""
+        lib().sfo_set_evolve_background(self.h, int(bool(on)))
+
+    def background(self):
+        """The current background velocity U_ (N3 values)."""
+        U = np.zeros(self.N3)
+        lib().sfo_get_background(self.h, _p(U))
+        return U
 
     def cfl(self, linear=False):
         """CFL (IMEXRK.h:355-374) or CFLlinear (IMEXRK.h:376-393) for the current dt,

@@ -235,6 +235,7 @@ public:
     std::vector<T> U1, U2, U3, B, U1_tot, U2_tot, U3_tot, B_tot, nnTemp, prod;
     std::vector<T> u1F, u2F, u3F, bF, nnTemp2, ndTemp, ndTemp2;  // IMEXRK.h:625-647
     std::vector<T> Ubar;  // Neumann1D U_ (IMEXRK.h:609)
+    bool evolveBackground = false;  // FlowParams::EvolveBackground (Parameters.h:29)
     std::vector<T> S;     // dB/dz profile at Neumann points (1 in the reference)
     double deltaT = 0.01;
     T h[3];
@@ -910,6 +911,20 @@ public:
         solve(R3, u3, 1, DIRICHLET);
         explicit_half(RB, b, NEUMANN, cb);
         solve(RB, b, 2, NEUMANN);
+        // IMEXRK.cpp:100-104: RU_ = U_ + (h/2Re) Dzz U_, ZeroEnds of the 1D
+        // field (slots 0, 1, N3-2, N3-1, Field.h:625-636), then the (0,0)
+        // velocity Neumann matrix of this stage (CNSolve1D, IMEXRK.h:584-588)
+        if (evolveBackground) {
+            std::vector<C> Uc(N3), t(N3), RU(N3);
+            for (int j = 0; j < N3; ++j) Uc[j] = C(Ubar[j]);
+            op_dzz(NEUMANN, Uc.data(), t.data());
+            for (int j = 0; j < N3; ++j) RU[j] = Uc[j] + cv * t[j];
+            RU[0] = RU[1] = RU[N3 - 2] = RU[N3 - 1] = C(0);
+            Tri tr;
+            build_tri(0, k, 0, 0, tr);
+            tri_solve(tr, RU.data(), Uc.data());
+            for (int j = 0; j < N3; ++j) Ubar[j] = Uc[j].real();
+        }
     }
 
     void stage(int k, bool linear) {
@@ -1468,6 +1483,21 @@ void sfo_adjoint(void* p, int mode, const double* const* below, const double* co
 }
 double sfo_jover_k(void* p) {
     return dispatch(static_cast<Handle*>(p), [&](auto& s) { return double(s.jover_k()); });
+}
+// FlowParams::EvolveBackground: U_ diffuses by one Crank-Nicolson solve per
+// stage (IMEXRK.cpp:100-104)
+void sfo_set_evolve_background(void* p, int on) {
+    dispatch(static_cast<Handle*>(p), [&](auto& s) {
+        s.evolveBackground = on != 0;
+        return 0;
+    });
+}
+// the current U_ (N3 values)
+void sfo_get_background(void* p, double* U) {
+    dispatch(static_cast<Handle*>(p), [&](auto& s) {
+        for (int j = 0; j < s.N3; ++j) U[j] = double(s.Ubar[j]);
+        return 0;
+    });
 }
 // CFL (linear == 0, IMEXRK.h:355-374) or CFLlinear (IMEXRK.h:376-393) for the
 // current dt, without its dt update

@@ -59,7 +59,7 @@ EXPORTED = [
     "sf_set_linear_background", "sf_step_adjoint", "sf_jover_k", "sf_field_add_profile", "sf_field_mean_profile",
     "sf_dot_f64", "sf_field_enforce_bcs", "sf_phase_shift", "sf_remove_average", "sf_state_bind", "sf_state_fetch",
     "sf_energy_f64", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
-    "sf_energy", "sf_cfl", "sf_cfl_linear", "sf_cfl_linear_noalloc", "sf_refresh_nodal", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
+    "sf_energy", "sf_cfl", "sf_cfl_linear", "sf_cfl_linear_noalloc", "sf_refresh_nodal", "sf_get_background", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
     "sf_batch_create", "sf_batch_destroy", "sf_batch_last_error", "sf_batch_step", "sf_batch_time_steps",
     "sf_batch_info",
     "sf_net_create", "sf_net_destroy", "sf_net_last_error", "sf_net_apply", "sf_net_add_pair", "sf_net_clear_pairs",
@@ -131,6 +131,7 @@ def lib():
         "sf_cfl_linear": [vp, dp],
         "sf_cfl_linear_noalloc": [vp, dp],
         "sf_refresh_nodal": [vp],
+        "sf_get_background": [vp, dp],
         "sf_synchronize": [vp],
         "sf_time_steps": [vp, i, dp],
         "sf_time_steps_detailed": [vp, i, dp, dp, ctypes.POINTER(ctypes.c_int)],
@@ -293,15 +294,15 @@ class IMEXRK:
     ``-DDOUBLE=On`` build.
     """
 
-    def __init__(self, N1, N2, N3, dimensionality, L1, L2, L3, Re, Ri, Pr, precision=4):
+    def __init__(self, N1, N2, N3, dimensionality, L1, L2, L3, Re, Ri, Pr, precision=4, evolve_background=False):
         L = lib()
         self.N1, self.N2, self.N3 = N1, N2, N3
         self.M1 = N1 // 2 + 1
         self.dimensionality = dimensionality
         self.precision = precision
-        self.flow = dict(L1=L1, L2=L2, L3=L3, Re=Re, Ri=Ri, Pr=Pr)
+        self.flow = dict(L1=L1, L2=L2, L3=L3, Re=Re, Ri=Ri, Pr=Pr, EvolveBackground=bool(evolve_background))
         g = _Grid(N1, N2, N3, dimensionality)
-        f = _Flow(L1, L2, L3, Re, Ri, Pr, 0)
+        f = _Flow(L1, L2, L3, Re, Ri, Pr, int(bool(evolve_background)))
         h = ctypes.c_void_p()
         rc = L.sf_ctx_create(ctypes.byref(g), ctypes.byref(f), precision, ctypes.byref(h))
         if rc != 0:
@@ -348,8 +349,15 @@ class IMEXRK:
     def ResetForParams(self, **flow):
         self.flow.update(flow)
         f = _Flow(self.flow["L1"], self.flow["L2"], self.flow["L3"], self.flow["Re"], self.flow["Ri"],
-                  self.flow["Pr"], 0)
+                  self.flow["Pr"], int(bool(self.flow["EvolveBackground"])))
         self._chk(lib().sf_set_flow(self.h, ctypes.byref(f)))
+
+    def Background(self):
+        """The background velocity U_ (N3 values) as the solver holds it now: it
+        changes every stage when EvolveBackground is set (IMEXRK.cpp:100-104)."""
+        U = np.zeros(self.N3)
+        self._chk(lib().sf_get_background(self.h, _p(U)))
+        return U
 
     def vertical_points(self):
         """(z, zf): VerticalPoints / VerticalPointsFractional (Field.cpp:3-31)."""

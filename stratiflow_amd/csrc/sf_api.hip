@@ -82,6 +82,7 @@ struct Ctx {
     int zch = 4;   // z levels per x-pass block
     int device = 0;  // the device current at creation
     double L1 = 0, L2 = 0, L3 = 0, Re = 0, Ri = 0, Pr = 0;
+    bool evolve_bg = false;  // FlowParams::EvolveBackground
     double cfl_d[3] = {1, 1, 1};  // L1/N1, L2/N2, z(N3/2+1) - z(N3/2) (IMEXRK.h:359-364)
     double dt = 0.01;
     T h[3] = {0, 0, 0};
@@ -168,7 +169,6 @@ struct Ctx {
         } else {
             SF_REQUIRE(N2 == 1, SF_ERR_ARG, "2D requires N2 == 1 (IMEXRK.h:43)");
         }
-        SF_REQUIRE(!f.evolve_background, SF_ERR_UNSUPPORTED, "EvolveBackground is not supported");
         SF_REQUIRE(!(N1 > 384 && sizeof(T) == 8), SF_ERR_UNSUPPORTED, "FP64 contexts support N1 <= 384");
         SF_HIP(hipGetDevice(&device));
         M1 = N1 / 2 + 1;
@@ -243,6 +243,9 @@ struct Ctx {
 
     void set_flow(const sf_flow& f) {
         SF_REQUIRE(f.L1 > 0 && f.L2 > 0 && f.L3 > 0 && f.Re > 0 && f.Pr > 0, SF_ERR_ARG, "bad flow parameters");
+        // the device holds U_ as it evolves; keep it across the re-upload below
+        if (zc && evolve_bg) fetch_background();
+        evolve_bg = f.evolve_background != 0;
         L1 = f.L1;
         L2 = f.L2;
         L3 = f.L3;
@@ -319,6 +322,14 @@ struct Ctx {
         tw(N1, tw1);
         tw(N2, tw2);
         SF_HIP(hipStreamSynchronize(stream));
+    }
+
+    // U_ from the device into the host copy of the coefficient table
+    void fetch_background() {
+        std::vector<T> u(N3);
+        SF_HIP(hipMemcpyAsync(u.data(), zc + size_t(ZC_UBAR) * N3p, N3 * sizeof(T), hipMemcpyDeviceToHost, stream));
+        SF_HIP(hipStreamSynchronize(stream));
+        for (int j = 0; j < N3; ++j) zc_host[size_t(ZC_UBAR) * N3p + j] = double(u[j]);
     }
 
     void set_background(const double* U, const double* S) {
@@ -471,6 +482,9 @@ struct Ctx {
         m.Ri = T(Ri);
         m.pmult = T(1) / h[k];
         m.mode = 0;
+        // CrankNicolson evolves U_ whenever the flag is set (IMEXRK.cpp:100-104),
+        // in the linearised and adjoint stages as well
+        m.ubar = evolve_bg ? zc + size_t(ZC_UBAR) * N3p : nullptr;
         return m;
     }
 
@@ -492,6 +506,7 @@ struct Ctx {
         m.u3 = mf(SF_U3);
         m.b = mf(SF_B);
         m.p = mf(SF_P);
+        m.ubar = nullptr;
         return m;
     }
 
@@ -1727,6 +1742,17 @@ int sf_cfl(sf_ctx* ctx, double* cfl) {
     SF_NULLCHK(ctx);
     SF_NULLCHK(cfl);
     return guarded(ctx, [&]() { with_ctx(ctx, [&](auto& C) { *cfl = C.cfl(); }); });
+}
+
+int sf_get_background(sf_ctx* ctx, double* U) {
+    SF_NULLCHK(ctx);
+    SF_NULLCHK(U);
+    return guarded(ctx, [&]() {
+        with_ctx(ctx, [&](auto& C) {
+            C.fetch_background();
+            for (int j = 0; j < C.N3; ++j) U[j] = C.zc_host[size_t(ZC_UBAR) * C.N3p + j];
+        });
+    });
 }
 
 int sf_cfl_linear_noalloc(sf_ctx* ctx, double* cfl) {

@@ -18,6 +18,7 @@ struct ModalArgs {
     T hz, mh, hb, cv, cb, Ri, pmult;
     int mode;         // 0: full stage, 1: projection only (RemoveDivergence)
     int skip_linear;  // 1: n1..nB are the complete explicit terms (BuildRHSAdjoint)
+    T* ubar;          // non-null: EvolveBackground, U_ (the ZC_UBAR row of zc) diffuses in place
 };
 
 // Position of element (row, z, j1) of every spectral array: row is the active

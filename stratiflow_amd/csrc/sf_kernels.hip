@@ -304,6 +304,28 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         tri_apply<T, M>(F, R, lane);
 #pragma unroll
         for (int t = 0; t < M; ++t) u1[t] = R[t];
+        // EvolveBackground (IMEXRK.cpp:100-104): the (0,0) wave, which holds the
+        // factor of that mode's velocity Neumann matrix (CNSolve1D, IMEXRK.h:584-588),
+        // diffuses U_ in place: RU = U_ + cv Dzz U_, ZeroEnds of the 1D field
+        // (Field.h:625-636), U_ = T^-1 RU.  No other wave of the launch reads U_.
+        if (A.ubar != nullptr && zero_mode) {  // wave-uniform
+            V ub[M], um[M], up[M];
+#pragma unroll
+            for (int t = 0; t < M; ++t) ub[t] = mkc<T>(coefz(ZC_UBAR, t), T(0));
+            nb_prev<T, M>(ub, um, lane);
+            nb_next<T, M>(ub, up, lane);
+#pragma unroll
+            for (int t = 0; t < M; ++t) {
+                const int j = row[t];
+                const V dzz = cadd(cadd(cscale(coefz(ZC_DZZN_LO, t), um[t]), cscale(coefz(ZC_DZZN_DI, t), ub[t])),
+                                   cscale(coefz(ZC_DZZN_UP, t), up[t]));
+                R[t] = (j <= 1 || j >= N - 2) ? czero<V>() : cadd(ub[t], cscale(A.cv, dzz));
+            }
+            tri_apply<T, M>(F, R, lane);
+#pragma unroll
+            for (int t = 0; t < M; ++t)
+                if (row[t] < N) A.ubar[row[t]] = R[t].x;
+        }
         if (three) {
 #pragma unroll
             for (int t = 0; t < M; ++t) gp[t] = cik(ky, pp[t]);

@@ -484,3 +484,53 @@ def test_linearised_rhs_is_the_derivative(dim, N2):
         fd = (rp[n] - rm[n]) / (2 * eps)
         lin = o.get_modal(n)
         assert np.linalg.norm(fd - lin) / np.linalg.norm(lin) < 1e-9, n
+
+
+# ---------------------------------------------------------------- EvolveBackground
+def _background_stage(U, N3=48, Re=700.0, dt=0.02):
+    o = make(N1=12, N3=N3, Re=Re)
+    o.set_background(U)
+    o.set_evolve_background(True)
+    o.set_dt(dt)
+    o.stage(0)  # zero state: only U_ changes
+    return o, o.background()
+
+
+def test_evolve_background_constant_profile():
+    """A uniform U_ is a steady solution of the interior rows; the Neumannified
+    boundary rows (Differentiation.h:17-35) put the ghost values to zero after
+    ZeroEnds of the 1D right-hand side (Field.h:625-636)."""
+    N3 = 48
+    _, U1 = _background_stage(np.full(N3, 0.7), N3)
+    assert np.allclose(U1[1:N3 - 1], 0.7, rtol=0, atol=1e-14)
+    assert U1[0] == 0.0 and U1[N3 - 1] == 0.0
+
+
+def test_evolve_background_matches_dense_crank_nicolson():
+    """One stage of IMEXRK.cpp:100-104 against a dense solve written from the
+    reference's definitions: RU = U + (h/2Re) Dzz U with h = dt*8/15, ZeroEnds
+    (slots 0, 1, N3-2, N3-1), then (I - (h/2Re) Dzz) Neumannified (rows 0 and
+    N3-1 identity, row 1 = e1 - e2, row N3-2 = e_{N3-3} - e_{N3-2}).  Dzz is the
+    oracle's banded operator, pinned on quadratics and as Dz o Dz above."""
+    N3, Re, dt = 48, 700.0, 0.02
+    o0 = make(N1=12, N3=N3)
+    z, zf, _, _ = o0.grid()
+    U = np.tanh(zf) + 0.3 * np.exp(-zf ** 2)
+    o, U1 = _background_stage(U, N3, Re, dt)
+    D = np.stack([o0.apply_op("dzz_N", e) for e in np.eye(N3)], axis=1)
+    c = 0.5 * (dt * 8.0 / 15.0) / Re
+    rhs = U + c * D @ U
+    rhs[[0, 1, N3 - 2, N3 - 1]] = 0
+    T = np.eye(N3) - c * D
+    T[0, :] = 0
+    T[0, 0] = 1
+    T[1, :] = 0
+    T[1, 1], T[1, 2] = 1, -1
+    T[N3 - 2, :] = 0
+    T[N3 - 2, N3 - 3], T[N3 - 2, N3 - 2] = 1, -1
+    T[N3 - 1, :] = 0
+    T[N3 - 1, N3 - 1] = 1
+    want = np.linalg.solve(T, rhs)
+    assert np.allclose(U1, want, rtol=0, atol=1e-13)
+    # and it diffuses: the interior profile moves toward its mean
+    assert np.abs(np.diff(U1[1:N3 - 1])).max() < np.abs(np.diff(U[1:N3 - 1])).max()
