@@ -215,7 +215,11 @@ constexpr int fft_ldp() {
     }
 }
 
-template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
+// ES: element stride of a line (1: contiguous lines; lines interleaved with
+// ES = line count when element i of line l sits at lp(l) + i*ES).  IL: work
+// items run over the lines first (consecutive lanes take consecutive lines,
+// which is what keeps interleaved lines free of LDS bank conflicts).
+template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, typename LP>
 __device__ __forceinline__ void fft_pass1(int nl, LP lp, const int tid) {
     using V = cx<T>;
     constexpr int R1 = FftPlan<N>::R1, R2 = FftPlan<N>::R2;
@@ -226,11 +230,11 @@ __device__ __forceinline__ void fft_pass1(int nl, LP lp, const int tid) {
     for (int i = 0; i < PER; ++i) {
         const int g = tid + i * BS;
         if (g < tot) {
-            const int line = g / R2;
-            const int j = g - line * R2;
-            const V* L = lp(line) + j;
+            const int line = IL ? g % nl : g / R2;
+            const int j = IL ? g / nl : g - line * R2;
+            const V* L = lp(line) + j * ES;
 #pragma unroll
-            for (int r = 0; r < R1; ++r) v[i][r] = L[R2 * r];
+            for (int r = 0; r < R1; ++r) v[i][r] = L[R2 * r * ES];
             reg_dft<T, R1, SIGN>(v[i]);
         }
     }
@@ -239,11 +243,11 @@ __device__ __forceinline__ void fft_pass1(int nl, LP lp, const int tid) {
     for (int i = 0; i < PER; ++i) {
         const int g = tid + i * BS;
         if (g < tot) {
-            const int line = g / R2;
-            const int j = g - line * R2;
-            V* Lp = lp(line) + j * (R1 + 1);
+            const int line = IL ? g % nl : g / R2;
+            const int j = IL ? g / nl : g - line * R2;
+            V* Lp = lp(line) + j * (R1 + 1) * ES;
 #pragma unroll
-            for (int r = 0; r < R1; ++r) Lp[r] = v[i][r];
+            for (int r = 0; r < R1; ++r) Lp[r * ES] = v[i][r];
         }
     }
     sync_lds();
@@ -251,7 +255,7 @@ __device__ __forceinline__ void fft_pass1(int nl, LP lp, const int tid) {
 
 // pass 2 twiddles: tw[j*R2 + r] = exp(-2 pi i r j / N), j < R1, r < R2 (each
 // lane reads its R2 - 1 twiddles from one contiguous run, in wide loads)
-template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
+template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, typename LP>
 __device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict__ tw, const int tid) {
     using V = cx<T>;
     constexpr int R1 = FftPlan<N>::R1, R2 = FftPlan<N>::R2;
@@ -262,12 +266,12 @@ __device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict
     for (int i = 0; i < PER; ++i) {
         const int g = tid + i * BS;
         if (g < tot) {
-            const int line = g / R1;
-            const int j = g - line * R1;
-            const V* Lp = lp(line) + j;
+            const int line = IL ? g % nl : g / R1;
+            const int j = IL ? g / nl : g - line * R1;
+            const V* Lp = lp(line) + j * ES;
             const V* twj = tw + j * R2;
 #pragma unroll
-            for (int r = 0; r < R2; ++r) v[i][r] = Lp[r * (R1 + 1)];
+            for (int r = 0; r < R2; ++r) v[i][r] = Lp[r * (R1 + 1) * ES];
 #pragma unroll
             for (int r = 1; r < R2; ++r) {
                 V w = twj[r];
@@ -282,17 +286,17 @@ __device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict
     for (int i = 0; i < PER; ++i) {
         const int g = tid + i * BS;
         if (g < tot) {
-            const int line = g / R1;
-            const int j = g - line * R1;
-            V* L = lp(line) + j;
+            const int line = IL ? g % nl : g / R1;
+            const int j = IL ? g / nl : g - line * R1;
+            V* L = lp(line) + j * ES;
 #pragma unroll
-            for (int r = 0; r < R2; ++r) L[R1 * r] = v[i][r];
+            for (int r = 0; r < R2; ++r) L[R1 * r * ES] = v[i][r];
         }
     }
     sync_lds();
 }
 
-template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
+template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, typename LP>
 __device__ __forceinline__ void fft_single(int nl, LP lp, const int tid) {
     using V = cx<T>;
     constexpr int PER = (NLMAX + BS - 1) / BS;
@@ -303,7 +307,7 @@ __device__ __forceinline__ void fft_single(int nl, LP lp, const int tid) {
         if (line < nl) {
             const V* L = lp(line);
 #pragma unroll
-            for (int r = 0; r < N; ++r) v[i][r] = L[r];
+            for (int r = 0; r < N; ++r) v[i][r] = L[r * ES];
             reg_dft<T, N, SIGN>(v[i]);
         }
     }
@@ -314,25 +318,26 @@ __device__ __forceinline__ void fft_single(int nl, LP lp, const int tid) {
         if (line < nl) {
             V* L = lp(line);
 #pragma unroll
-            for (int r = 0; r < N; ++r) L[r] = v[i][r];
+            for (int r = 0; r < N; ++r) L[r * ES] = v[i][r];
         }
     }
     sync_lds();
 }
 
 // In-place FFT of nl <= NLMAX lines in LDS, each in natural order at lp(line)
-// with room for fft_ldp<N>() values.  The caller must have synchronised after
+// with room for fft_ldp<N>() values (element stride ES: room for that many
+// positions, N plus the pass-1 padding of R2).  The caller must have synchronised after
 // filling the lines; on return the results are visible to all threads.  tid
 // is the thread's index in [0, BS) for the distribution of the work items (a
 // block may permute it to spread partially filled rounds over its waves).
-template <typename T, int N, int NLMAX, int BS, int SIGN, typename LP>
+template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, typename LP>
 __device__ __forceinline__ void lds_fft_lines(int nl, LP lp, const cx<T>* __restrict__ tw,
                                               const int tid = int(threadIdx.x)) {
     if constexpr (FftPlan<N>::two) {
-        fft_pass1<T, N, NLMAX, BS, SIGN>(nl, lp, tid);
-        fft_pass2<T, N, NLMAX, BS, SIGN>(nl, lp, tw, tid);
+        fft_pass1<T, N, NLMAX, BS, SIGN, ES, IL>(nl, lp, tid);
+        fft_pass2<T, N, NLMAX, BS, SIGN, ES, IL>(nl, lp, tw, tid);
     } else {
-        fft_single<T, N, NLMAX, BS, SIGN>(nl, lp, tid);
+        fft_single<T, N, NLMAX, BS, SIGN, ES>(nl, lp, tid);
         (void)tw;
     }
 }

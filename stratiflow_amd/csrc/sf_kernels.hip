@@ -1082,6 +1082,121 @@ __global__ __launch_bounds__(BS) void k_yfwd(YArgs<T> A) {
     }
 }
 
+// Paired forward y transform with the rows loaded straight into LDS
+// (buffer_load ... lds, no VGPR staging), one block per (z, tile of YJ j1).
+// Each of the two fields of a pair arrives as N2 rows of YJ consecutive j1,
+// row pitch YJ: the YJ lines of a field are interleaved (element n2 of line
+// jj at n2*YJ + jj) and the FFT runs over them line-fastest, so lanes touch
+// consecutive LDS words.  Global reads and writes move runs of YJ values.
+template <typename T, int N2, int YJ>
+struct YDma {
+    using V = cx<T>;
+    static constexpr int R2 = FftPlan<N2>::two ? FftPlan<N2>::R2 : 1;
+    static constexpr int ROWS = N2 + R2;                   // positions per line, pass-1 padding included
+    static constexpr int FIELD = ROWS * YJ;                // values per field region
+    static constexpr int VPL = 16 / int(sizeof(V));        // values per lane per 16-byte load
+    static constexpr int LPR = YJ / VPL;                   // lanes per row
+    static constexpr int RPI = 64 / LPR;                   // rows per wave load
+    static constexpr size_t LDS = size_t(2) * FIELD * sizeof(V);
+    static bool fits(int A1, int nin) {
+        return N2 % RPI == 0 && (A1 * int(sizeof(V))) % 16 == 0 && nin % 2 == 0 && LDS <= 160 * 1024;
+    }
+};
+
+template <typename T, int N2, int YJ>
+__global__ __launch_bounds__(BS) void k_yfwd_dma(YArgs<T> A) {
+    using V = cx<T>;
+    using D = YDma<T, N2, YJ>;
+    __shared__ __align__(16) V buf[2 * D::FIELD];
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int z = tile / gridDim.y;
+    const int j10 = (tile % gridDim.y) * YJ;
+    const int N3 = A.N3, A1 = A.A1, A2 = A.A2;
+    const int h = N2 / 3;
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    const size_t fbytes = size_t(N2) * size_t(N3) * size_t(A1) * sizeof(V);
+    // this lane's element of a row load: row r0 + lane/LPR, j1 = j10 + col
+    const int col = (lane % D::LPR) * D::VPL;
+    const int lrow = lane / D::LPR;
+    const bool jok = j10 + col < A1;
+    for (int f0 = 0; f0 < 2 * A.nf; f0 += 2) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const rsrc_t r = make_rsrc(A.in[f0 + q], fbytes);
+            for (int i = wave; i < N2 / D::RPI; i += BS / 64) {
+                const int row = i * D::RPI + lrow;
+                const int off = jok ? int(sidx(row, z, j10 + col, N3, A1) * sizeof(V)) : BUF_OOB;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    r, (__attribute__((address_space(3))) void*)(buf + q * D::FIELD + i * D::RPI * YJ), 16, off, 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        lds_fft_lines<T, N2, 2 * YJ, BS, -1, YJ, true>(
+            2 * YJ, [&](int line) { return buf + (line / YJ) * D::FIELD + (line % YJ); }, A.tw);
+        const int rr = f0 / 2;
+        const rsrc_t o = make_rsrc(A.out[rr], size_t(A2) * size_t(N3) * size_t(A1) * sizeof(V));
+        for (int idx = threadIdx.x; idx < A2 * YJ; idx += BS) {
+            const int jj = idx % YJ;
+            const int ja = idx / YJ;
+            const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
+            const V Fa = buf[j2 * YJ + jj];
+            const V Fb = buf[D::FIELD + j2 * YJ + jj];
+            const int off = (j10 + jj < A1) ? int(sidx(ja, z, j10 + jj, N3, A1) * sizeof(V)) : BUF_OOB;
+            bst<V>(cscale(A.scale, cadd(Fa, cik(A.ky[ja], Fb))), o, off);
+        }
+        __syncthreads();  // every wave is done with buf before the next pair's loads
+    }
+}
+
+// Inverse y transform with the modal rows loaded straight into LDS, one block
+// per (z, tile of YJ j1), the nf fields one after the other.  Row ja of a
+// field lands at its line position j2(ja) (interleaved lines as in
+// k_yfwd_dma); the dealiased band j2 = h .. N2-h is cleared in LDS.  A wave
+// load covers RPI consecutive rows, so h must be a multiple of RPI for no load
+// to straddle the band.
+template <typename T, int N2, int YJ>
+__global__ __launch_bounds__(BS) void k_yinv_dma(YArgs<T> A) {
+    using V = cx<T>;
+    using D = YDma<T, N2, YJ>;
+    __shared__ __align__(16) V buf[D::FIELD];
+    const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int z = tile / gridDim.y;
+    const int j10 = (tile % gridDim.y) * YJ;
+    const int N3 = A.N3, A1 = A.A1, A2 = A.A2;
+    const int h = N2 / 3;
+    const int nband = N2 - 2 * h + 1;
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    const int col = (lane % D::LPR) * D::VPL;
+    const int lrow = lane / D::LPR;
+    const bool jok = j10 + col < A1;
+    const size_t mbytes = size_t(A2) * size_t(N3) * size_t(A1) * sizeof(V);
+    const size_t ybytes = size_t(N2) * size_t(N3) * size_t(A1) * sizeof(V);
+    for (int f = 0; f < A.nf; ++f) {
+        for (int idx = threadIdx.x; idx < nband * YJ; idx += BS) buf[(h + idx / YJ) * YJ + idx % YJ] = czero<V>();
+        const rsrc_t r = make_rsrc(A.in[f], mbytes);
+        for (int i = wave; i * D::RPI < A2; i += BS / 64) {
+            const int ja0 = i * D::RPI;
+            const int ja = ja0 + lrow;
+            const int pos0 = (ja0 < h) ? ja0 : ja0 + nband;
+            const int off = (jok && ja < A2) ? int(sidx(ja, z, j10 + col, N3, A1) * sizeof(V)) : BUF_OOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)(buf + pos0 * YJ), 16,
+                                                     off, 0, 0, 0);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        lds_fft_lines<T, N2, YJ, BS, +1, YJ, true>(YJ, [&](int line) { return buf + line; }, A.tw);
+        const rsrc_t o = make_rsrc(A.out[f], ybytes);
+        for (int idx = threadIdx.x; idx < N2 * YJ; idx += BS) {
+            const int jj = idx % YJ;
+            const int n2 = idx / YJ;
+            const int off = (j10 + jj < A1) ? int(sidx(n2, z, j10 + jj, N3, A1) * sizeof(V)) : BUF_OOB;
+            bst<V>(buf[n2 * YJ + jj], o, off);
+        }
+        __syncthreads();  // every wave is done with buf before the next field's loads
+    }
+}
+
 template <typename T, int N2, int YJ, int FR>
 static hipError_t launch_y_v(const YArgs<T>& a, bool inv, hipStream_t s) {
     constexpr int lds = YJ * FR * y_ldp<N2, YJ>() * int(sizeof(cx<T>));
@@ -1097,10 +1212,31 @@ static hipError_t launch_y_v(const YArgs<T>& a, bool inv, hipStream_t s) {
 
 template <typename T, int N2>
 static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
-    // 16 lines per block keeps registers and LDS low enough for several
-    // blocks per CU: 16 j1 x 1 field, or 8 j1 x 2 fields for the paired
-    // forward pass
+    // the inverse loads 32-wide tiles (256-byte runs) straight into LDS, the
+    // paired forward pass two 16-wide fields, where the grid allows it; else
+    // 16 lines per block through registers: 16 j1 x 1 field (inverse and
+    // single fields) or 8 j1 x 2 fields (paired forward)
+    if constexpr (FftPlan<N2>::two && YDma<T, N2, 32>::LDS / 2 <= 160 * 1024) {
+        using D = YDma<T, N2, 32>;
+        // the last row load may run RPI-1 rows past A2: those land in the
+        // pass-1 padding rows, which must hold them
+        const int last = (a.A2 + D::RPI - 1) / D::RPI * D::RPI + (N2 - 2 * (N2 / 3) + 1);
+        if (inv && N2 % D::RPI == 0 && (N2 / 3) % D::RPI == 0 && last <= D::ROWS &&
+            (a.A1 * int(sizeof(cx<T>))) % 16 == 0) {
+            dim3 grid(a.N3, (a.A1 + 31) / 32);
+            hipLaunchKernelGGL((k_yinv_dma<T, N2, 32>), grid, dim3(BS), 0, s, a);
+            return hipGetLastError();
+        }
+    }
     if (inv || !a.paired) return launch_y_v<T, N2, 16, 1>(a, inv, s);
+    if constexpr (FftPlan<N2>::two && YDma<T, N2, 16>::LDS <= 160 * 1024) {
+        using D = YDma<T, N2, 16>;
+        if (D::fits(a.A1, 2 * a.nf)) {
+            dim3 grid(a.N3, (a.A1 + 15) / 16);
+            hipLaunchKernelGGL((k_yfwd_dma<T, N2, 16>), grid, dim3(BS), 0, s, a);
+            return hipGetLastError();
+        }
+    }
     return launch_y_v<T, N2, 8, 2>(a, inv, s);
 }
 
