@@ -747,7 +747,7 @@ public:
                          const NeumannModal& bBelow, const NeumannModal& u1Above, const NeumannModal& u2Above,
                          const DirichletModal& u3Above, const NeumannModal& bAbove) {
         sync_dt();
-        const bool three = gridParams.dimensionality == Dimensionality::ThreeDimensional;
+        const bool three = gridParams.ThirdDimension();
         const sf_field* below[4] = {u1Below.dev(false), three ? u2Below.dev(false) : nullptr, u3Below.dev(false),
                                     bBelow.dev(false)};
         const sf_field* above[4] = {u1Above.dev(false), three ? u2Above.dev(false) : nullptr, u3Above.dev(false),
@@ -813,9 +813,7 @@ public:
     void SetBackground(const NeumannModal& velocity1, const NeumannModal& velocity2, const DirichletModal& velocity3,
                        const NeumannModal& buoyancy) {
         sfacade::check(sf_set_linear_background(c(), velocity1.dev(false),
-                                                gridParams.dimensionality == Dimensionality::ThreeDimensional
-                                                    ? velocity2.dev(false)
-                                                    : nullptr,
+                                                gridParams.ThirdDimension() ? velocity2.dev(false) : nullptr,
                                                 velocity3.dev(false), buoyancy.dev(false)),
                        c());
     }

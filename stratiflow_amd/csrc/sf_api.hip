@@ -158,9 +158,11 @@ struct Ctx {
         N2 = g.N2;
         N3 = g.N3;
         dim = g.dimensionality;
-        SF_REQUIRE(dim == SF_THREE_D || dim == SF_TWO_D, SF_ERR_UNSUPPORTED,
-                   "dimensionality must be SF_THREE_D or SF_TWO_D");
-        three = (dim == SF_THREE_D);
+        SF_REQUIRE(dim == SF_THREE_D || dim == SF_TWO_D || dim == SF_TWO_AND_A_HALF, SF_ERR_ARG,
+                   "dimensionality must be SF_THREE_D, SF_TWO_D or SF_TWO_AND_A_HALF");
+        // TwoAndAHalf resolves the spanwise direction like ThreeDimensional
+        // (GridParams::ThirdDimension, Parameters.h:13-17)
+        three = (dim != SF_TWO_D);
         SF_REQUIRE(N3 >= 8 && N3 % 4 == 0 && N3 <= 64 * max_rows_per_lane(), SF_ERR_ARG,
                    "N3 must be a multiple of 4 in [8, 512] (Field.cpp:7)");
         SF_REQUIRE(tz_supported(N1), SF_ERR_UNSUPPORTED, "unsupported N1 (FFT sizes: 6..512 of the form 2^a*3^b)");
@@ -1310,7 +1312,6 @@ int sf_set_flow(sf_ctx* ctx, const sf_flow* flow) {
     SF_NULLCHK(flow);
     return guarded(ctx, [&]() {
         with_ctx(ctx, [&](auto& C) {
-            SF_REQUIRE(!flow->evolve_background, SF_ERR_UNSUPPORTED, "EvolveBackground is not supported");
             C.set_flow(*flow);
         });
     });

@@ -564,6 +564,12 @@ constexpr int xpass_waves() {
     return (std::is_same<T, float>::value && THREED && !LIN) ? 4 : 1;
 }
 
+// threads per x-pass block (a multiple of 64)
+template <typename T, bool THREED, bool LIN>
+constexpr int xpass_bs() {
+    return BS;
+}
+
 template <typename T, int N1, bool THREED, bool LIN>
 __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, const int n2) {
     using V = cx<T>;
@@ -572,7 +578,8 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
     constexpr int NP = THREED ? 6 : 3;     // product pair lines per level
     constexpr int NB = NP + NI;            // lines per batched FFT
     constexpr int AMAX = N1 / 3;           // A1 <= N1/3
-    constexpr int FI = (NI * AMAX + BS - 1) / BS;
+    constexpr int XB = xpass_bs<T, THREED, LIN>();
+    constexpr int FI = (NI * AMAX + XB - 1) / XB;
     // [NP product lines][ring slot 0: NI lines][slot 1][slot 2]
     __shared__ V lds[(NP + 3 * NI) * LDP];
     V* const prd = lds;
@@ -585,7 +592,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
     // the FFT passes, the unpacking of A1 modes) fall to the first waves of
     // the work-item order; rotating that order by whole waves from block to
     // block spreads the extra rounds over the SIMDs of a CU.
-    const int tid = (int(threadIdx.x) + 64 * ((bx + n2) & (BS / 64 - 1))) & (BS - 1);
+    const int tid = (int(threadIdx.x) + 64 * ((bx + n2) % (XB / 64))) % XB;
     auto slot = [&](int z) { return (z + 1 - zbeg) % 3; };
 
     // spectra of one level into registers: item = pair * A1 + j1
@@ -593,7 +600,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
     auto load = [&](int z) {
 #pragma unroll
         for (int i = 0; i < FI; ++i) {
-            const int item = tid + i * BS;
+            const int item = tid + i * XB;
             X[i] = czero<V>();
             Y[i] = czero<V>();
             if (item < NI * A1 && z >= 0 && z < N3) {
@@ -617,13 +624,13 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
     auto store = [&](int s) {
         V* R = ring + s * NI * LDP;
         const int nband = N1 - 2 * A1 + 1;  // j = A1 .. N1-A1 is zero
-        for (int idx = tid; idx < NI * nband; idx += BS) {
+        for (int idx = tid; idx < NI * nband; idx += XB) {
             const int pair = idx / nband;
             R[pair * LDP + A1 + (idx - pair * nband)] = czero<V>();
         }
 #pragma unroll
         for (int i = 0; i < FI; ++i) {
-            const int item = tid + i * BS;
+            const int item = tid + i * XB;
             if (item < NI * A1) {
                 const int pair = item / A1;
                 const int j1 = item - pair * A1;
@@ -648,7 +655,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
     // the previous level's store, consumed by this level's store
     if (zbeg + 1 < zend) load(zbeg + 2);
     sync_lds();
-    lds_fft_lines<T, N1, 3 * NI, BS, -1>(3 * NI, [ring](int line) { return ring + line * LDP; }, A.tw, tid);
+    lds_fft_lines<T, N1, 3 * NI, XB, -1>(3 * NI, [ring](int line) { return ring + line * LDP; }, A.tw, tid);
 
     for (int z = zbeg; z < zend; ++z) {
         const bool more = z + 1 < zend;
@@ -668,7 +675,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
             const V* Ls[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) Ls[d] = ring + slot(z - 1 + d) * NI * LDP;
-            for (int n = tid; n < N1; n += BS) {
+            for (int n = tid; n < N1; n += XB) {
                 T u1[3], u2[3], u3[3], bb[3], w1[3], w2[3], w3[3], wb[3];
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -753,14 +760,14 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
         sync_lds();
         {
             const int s2 = slot(z + 2) * NI;
-            lds_fft_lines<T, N1, NB, BS, -1>(more ? NB : NP,
+            lds_fft_lines<T, N1, NB, XB, -1>(more ? NB : NP,
                                              [prd, s2](int line) {
                                                  return prd + (line + (line >= NP ? s2 : 0)) * LDP;
                                              },
                                              A.tw, tid);
         }
         // ---- separate the two real spectra of each product line, combine ---
-        for (int k = tid; k < A1; k += BS) {
+        for (int k = tid; k < A1; k += XB) {
             V Fa[NP], Fb[NP];
 #pragma unroll
             for (int q = 0; q < NP; ++q) {
@@ -793,7 +800,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
 }
 
 template <typename T, int N1, bool THREED, bool LIN>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(xpass_waves<T, THREED, LIN>()))) void k_xpass(
+__global__ __launch_bounds__((xpass_bs<T, THREED, LIN>())) __attribute__((amdgpu_waves_per_eu(xpass_waves<T, THREED, LIN>()))) void k_xpass(
     XArgs<T> A) {
     xpass_body<T, N1, THREED, LIN>(A, blockIdx.x, blockIdx.y);
 }
@@ -803,6 +810,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(xpass_waves<
 template <typename T, int N1>
 __global__ __launch_bounds__(BS) void k_xpass_batch(const XArgs<T>* __restrict__ args, const int* __restrict__ start,
                                                     int nruns, int total) {
+    static_assert(xpass_bs<T, false, false>() == BS, "the batched 2D x-pass runs blocks of BS threads");
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const int r = find_run(start, nruns, item);
         xpass_body<T, N1, false, false>(args[r], item - start[r], 0);
@@ -834,7 +842,7 @@ static hipError_t launch_xpass_v(const XArgs<T>& a, int N2, hipStream_t s) {
         if (a.zch < 1 || a.A1 > N1 / 3) return hipErrorInvalidValue;
         const int nblk = (a.N3 + a.zch - 1) / a.zch;
         dim3 grid(nblk, THREED ? N2 : 1);
-        hipLaunchKernelGGL((k_xpass<T, N1, THREED, LIN>), grid, dim3(BS), 0, s, a);
+        hipLaunchKernelGGL((k_xpass<T, N1, THREED, LIN>), grid, dim3(xpass_bs<T, THREED, LIN>()), 0, s, a);
         return hipGetLastError();
     }
 }
@@ -866,7 +874,7 @@ static int xpass_resident(int ncu) {
         return 0;
     } else {
         int per = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_xpass<T, N1, THREED, LIN>, BS, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_xpass<T, N1, THREED, LIN>, xpass_bs<T, THREED, LIN>(), 0) != hipSuccess)
             per = 1;
         return (per < 1 ? 1 : per) * ncu;
     }

@@ -17,7 +17,7 @@ field u3 at VerticalPoints (Field.h:862-871).
 """
 import numpy as np
 
-THREE_D, TWO_D = 0, 1
+THREE_D, TWO_D, TWO_AND_A_HALF = 0, 1, 2
 
 
 def hermite(h, x):
@@ -32,7 +32,7 @@ def hermite(h, x):
 
 def active_j2(N2, dim):
     """Active spanwise indices in the reference's order (Field.h:1129-1166)."""
-    if dim != THREE_D or N2 == 1:
+    if dim == TWO_D or N2 == 1:
         return list(range(N2))
     h = N2 // 3
     return list(range(h)) + list(range(N2 - h + 1, N2))
@@ -70,7 +70,7 @@ def perturbation(N1, N2, N3, dim, z, zf, seed, amplitude=1e-3, width=2.0, fields
     A1 = N1 // 3
     js = active_j2(N2, dim)
     if fields is None:
-        fields = ["u1", "u2", "u3", "b"] if dim == THREE_D else ["u1", "u3", "b"]
+        fields = ["u1", "u2", "u3", "b"] if dim != TWO_D else ["u1", "u3", "b"]
     rng = np.random.Generator(np.random.PCG64(seed))
     out = {}
     for name in fields:

@@ -10,11 +10,13 @@ from helpers import LX, fields, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _pair(N1, N2, N3, dim, prec, seed=5, Re=800.0):
+def _pair(N1, N2, N3, dim, prec, seed=5, Re=800.0, via_reset=False):
     import oracle
     import stratiflow_amd as sf
     from stratiflow_amd import ic
-    s = sf.IMEXRK(N1, N2, N3, dim, LX, LX / 2, 10.0, Re, 0.12, 1.0, precision=prec, evolve_background=True)
+    s = sf.IMEXRK(N1, N2, N3, dim, LX, LX / 2, 10.0, Re, 0.12, 1.0, precision=prec, evolve_background=not via_reset)
+    if via_reset:  # switched on mid-run through FlowParams (sf_set_flow)
+        s.ResetForParams(EvolveBackground=True)
     o = oracle.Oracle(N1, N2, N3, dim, LX, LX / 2, 10.0, Re, 0.12, 1.0, precision=8)
     o.set_evolve_background(True)
     z, zf = s.vertical_points()
@@ -64,4 +66,18 @@ def test_evolve_background_steps_match_oracle(shape, prec, tol):
     for f in fields(dim):
         e = rel_l2(s.get(f), o.get_modal(f))
         assert e < (tol if f != "p" else 10 * tol), f
+    s.close()
+
+
+def test_evolve_background_switched_on_through_flow_params():
+    """EvolveBackground set after construction (IMEXRK::ResetForParams with
+    the new FlowParams) diffuses U_ exactly as when it is set at construction."""
+    s, o, U0 = _pair(48, 1, 64, 1, 8, via_reset=True)
+    s.TimeStep(2)
+    o.step(2)
+    Uo = o.background()
+    assert np.abs(Uo - U0).max() > 1e-5
+    assert np.abs(s.Background() - Uo).max() / np.abs(Uo).max() < 1e-13
+    for f in fields(1):
+        assert rel_l2(s.get(f), o.get_modal(f)) < (1e-10 if f != "p" else 1e-9), f
     s.close()
