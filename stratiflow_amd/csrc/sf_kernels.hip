@@ -1090,12 +1090,12 @@ __global__ __launch_bounds__(BS) void k_yfwd(YArgs<T> A) {
     }
 }
 
-// Paired forward y transform with the rows loaded straight into LDS
-// (buffer_load ... lds, no VGPR staging), one block per (z, tile of YJ j1).
-// Each of the two fields of a pair arrives as N2 rows of YJ consecutive j1,
-// row pitch YJ: the YJ lines of a field are interleaved (element n2 of line
-// jj at n2*YJ + jj) and the FFT runs over them line-fastest, so lanes touch
-// consecutive LDS words.  Global reads and writes move runs of YJ values.
+// y transforms with the rows loaded straight into LDS (buffer_load ... lds,
+// no VGPR staging), one block per (z, tile of YJ j1).  A field arrives as N2
+// rows of YJ consecutive j1, row pitch YJ: the YJ lines are interleaved
+// (element n2 of line jj at n2*YJ + jj) and the FFT runs over them
+// line-fastest, so lanes touch consecutive LDS words.  Global reads and writes
+// move runs of YJ values.  One field region of LDS per block.
 template <typename T, int N2, int YJ>
 struct YDma {
     using V = cx<T>;
@@ -1105,17 +1105,20 @@ struct YDma {
     static constexpr int VPL = 16 / int(sizeof(V));        // values per lane per 16-byte load
     static constexpr int LPR = YJ / VPL;                   // lanes per row
     static constexpr int RPI = 64 / LPR;                   // rows per wave load
-    static constexpr size_t LDS = size_t(2) * FIELD * sizeof(V);
+    static constexpr size_t LDS = size_t(FIELD) * sizeof(V);
     static bool fits(int A1, int nin) {
         return N2 % RPI == 0 && (A1 * int(sizeof(V))) % 16 == 0 && nin % 2 == 0 && LDS <= 160 * 1024;
     }
 };
 
+// Paired forward y transform: F(A) is kept in registers at the active rows
+// while B is loaded into the same LDS.
 template <typename T, int N2, int YJ>
-__global__ __launch_bounds__(BS) void k_yfwd_dma(YArgs<T> A) {
+__global__ __launch_bounds__(BS) void k_yfwd_seq(YArgs<T> A) {
     using V = cx<T>;
     using D = YDma<T, N2, YJ>;
-    __shared__ __align__(16) V buf[2 * D::FIELD];
+    constexpr int PI = (2 * (N2 / 3) * YJ + BS - 1) / BS;  // active rows x YJ per thread
+    __shared__ __align__(16) V buf[D::FIELD];
     const int tile = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
     const int z = tile / gridDim.y;
     const int j10 = (tile % gridDim.y) * YJ;
@@ -1123,11 +1126,11 @@ __global__ __launch_bounds__(BS) void k_yfwd_dma(YArgs<T> A) {
     const int h = N2 / 3;
     const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
     const size_t fbytes = size_t(N2) * size_t(N3) * size_t(A1) * sizeof(V);
-    // this lane's element of a row load: row r0 + lane/LPR, j1 = j10 + col
     const int col = (lane % D::LPR) * D::VPL;
     const int lrow = lane / D::LPR;
     const bool jok = j10 + col < A1;
     for (int f0 = 0; f0 < 2 * A.nf; f0 += 2) {
+        V Fa[PI];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const rsrc_t r = make_rsrc(A.in[f0 + q], fbytes);
@@ -1135,32 +1138,43 @@ __global__ __launch_bounds__(BS) void k_yfwd_dma(YArgs<T> A) {
                 const int row = i * D::RPI + lrow;
                 const int off = jok ? int(sidx(row, z, j10 + col, N3, A1) * sizeof(V)) : BUF_OOB;
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    r, (__attribute__((address_space(3))) void*)(buf + q * D::FIELD + i * D::RPI * YJ), 16, off, 0, 0, 0);
+                    r, (__attribute__((address_space(3))) void*)(buf + i * D::RPI * YJ), 16, off, 0, 0, 0);
             }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            lds_fft_lines<T, N2, YJ, BS, -1, YJ, true>(YJ, [&](int line) { return buf + line; }, A.tw);
+            if (q == 0) {
+#pragma unroll
+                for (int i = 0; i < PI; ++i) {
+                    const int idx = int(threadIdx.x) + i * BS;
+                    const int ja = idx / YJ;
+                    const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
+                    Fa[i] = (idx < A2 * YJ) ? buf[j2 * YJ + idx % YJ] : czero<V>();
+                }
+            } else {
+                const rsrc_t o = make_rsrc(A.out[f0 / 2], size_t(A2) * size_t(N3) * size_t(A1) * sizeof(V));
+#pragma unroll
+                for (int i = 0; i < PI; ++i) {
+                    const int idx = int(threadIdx.x) + i * BS;
+                    if (idx < A2 * YJ) {
+                        const int jj = idx % YJ;
+                        const int ja = idx / YJ;
+                        const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
+                        const V Fb = buf[j2 * YJ + jj];
+                        const int off = (j10 + jj < A1) ? int(sidx(ja, z, j10 + jj, N3, A1) * sizeof(V)) : BUF_OOB;
+                        bst<V>(cscale(A.scale, cadd(Fa[i], cik(A.ky[ja], Fb))), o, off);
+                    }
+                }
+            }
+            __syncthreads();  // every wave is done with buf before the next loads
         }
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        lds_fft_lines<T, N2, 2 * YJ, BS, -1, YJ, true>(
-            2 * YJ, [&](int line) { return buf + (line / YJ) * D::FIELD + (line % YJ); }, A.tw);
-        const int rr = f0 / 2;
-        const rsrc_t o = make_rsrc(A.out[rr], size_t(A2) * size_t(N3) * size_t(A1) * sizeof(V));
-        for (int idx = threadIdx.x; idx < A2 * YJ; idx += BS) {
-            const int jj = idx % YJ;
-            const int ja = idx / YJ;
-            const int j2 = (ja < h) ? ja : ja + N2 - 2 * h + 1;
-            const V Fa = buf[j2 * YJ + jj];
-            const V Fb = buf[D::FIELD + j2 * YJ + jj];
-            const int off = (j10 + jj < A1) ? int(sidx(ja, z, j10 + jj, N3, A1) * sizeof(V)) : BUF_OOB;
-            bst<V>(cscale(A.scale, cadd(Fa, cik(A.ky[ja], Fb))), o, off);
-        }
-        __syncthreads();  // every wave is done with buf before the next pair's loads
     }
 }
 
 // Inverse y transform with the modal rows loaded straight into LDS, one block
 // per (z, tile of YJ j1), the nf fields one after the other.  Row ja of a
 // field lands at its line position j2(ja) (interleaved lines as in
-// k_yfwd_dma); the dealiased band j2 = h .. N2-h is cleared in LDS.  A wave
+// k_yfwd_seq); the dealiased band j2 = h .. N2-h is cleared in LDS.  A wave
 // load covers RPI consecutive rows, so h must be a multiple of RPI for no load
 // to straddle the band.
 template <typename T, int N2, int YJ>
@@ -1221,10 +1235,12 @@ static hipError_t launch_y_v(const YArgs<T>& a, bool inv, hipStream_t s) {
 template <typename T, int N2>
 static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
     // the inverse loads 32-wide tiles (256-byte runs) straight into LDS, the
-    // paired forward pass two 16-wide fields, where the grid allows it; else
-    // 16 lines per block through registers: 16 j1 x 1 field (inverse and
-    // single fields) or 8 j1 x 2 fields (paired forward)
-    if constexpr (FftPlan<N2>::two && YDma<T, N2, 32>::LDS / 2 <= 160 * 1024) {
+    // paired forward pass 16-wide tiles of one field at a time (LDS for six
+    // blocks per CU in FP32; 32-wide tiles, or both fields of a pair in LDS,
+    // measured no faster), where the grid allows it; else 16 lines per block
+    // through registers: 16 j1 x 1 field (inverse and single fields) or 8 j1 x
+    // 2 fields (paired forward)
+    if constexpr (FftPlan<N2>::two && YDma<T, N2, 32>::LDS <= 160 * 1024) {
         using D = YDma<T, N2, 32>;
         // the last row load may run RPI-1 rows past A2: those land in the
         // pass-1 padding rows, which must hold them
@@ -1241,7 +1257,7 @@ static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
         using D = YDma<T, N2, 16>;
         if (D::fits(a.A1, 2 * a.nf)) {
             dim3 grid(a.N3, (a.A1 + 15) / 16);
-            hipLaunchKernelGGL((k_yfwd_dma<T, N2, 16>), grid, dim3(BS), 0, s, a);
+            hipLaunchKernelGGL((k_yfwd_seq<T, N2, 16>), grid, dim3(BS), 0, s, a);
             return hipGetLastError();
         }
     }
