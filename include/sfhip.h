@@ -265,9 +265,13 @@ int sf_time_steps_detailed(sf_ctx* ctx, int nsteps, double* total_ms, double* ke
  * singleton, StateVector.h:546); an ensemble of independent runs is a loop of
  * FullEvolve/FixedEvolve calls, each TimeStep (IMEXRK.cpp:8-24) on its own.
  * A batch advances many independent 2D contexts (any N1, N3, Re, Ri, Pr, dt)
- * together: runs with the same N1 share each x-pass launch and runs with the
- * same vertical tile share each modal launch, and one step of all runs is a
- * HIP graph replay.  Each context keeps its own state, set up through the
+ * together, and one step of all runs is a HIP graph replay.  Each phase of a
+ * stage runs under one of two plans: one launch per distinct N1 (x-pass) or
+ * per vertical tile (modal), or ONE launch for all runs (SF_BATCH_MIXED; the
+ * modal kernel with 4 or 8 modes per block).  sf_batch_create times every
+ * plan, with its levels per block and resident blocks, and keeps the fastest
+ * (sf_batch_plan reports the choice and the timings); every plan gives the
+ * same results bit for bit.  Each context keeps its own state, set up through the
  * usual calls (sf_set_background, uploads, sf_filter_all, sf_populate_nodal,
  * sf_remove_divergence, sf_set_dt); sf_batch_step(b, n) then has the effect of
  * sf_step(ctx, n) on every member.  The member contexts must outlive the
@@ -280,6 +284,14 @@ int sf_batch_step(sf_batch* b, int nsteps);
 /* as sf_batch_step, returning the GPU time of the graph replays */
 int sf_batch_time_steps(sf_batch* b, int nsteps, double* total_ms);
 int sf_batch_info(const sf_batch* b, int* launches_per_step, int* xpass_groups, int* modal_groups);
+#define SF_BATCH_GROUPED 0 /* one launch per N1 (x-pass) or per vertical tile (modal) */
+#define SF_BATCH_MIXED 1   /* one launch for all runs (modal: 4 modes per block) */
+#define SF_BATCH_MIXED8 2  /* modal only: one launch, 8 modes per block */
+/* the plans in use and the tuner's GPU times per stage in ms: x-pass grouped,
+ * x-pass mixed, modal grouped, modal mixed (4), modal mixed (8) */
+int sf_batch_plan(const sf_batch* b, int* xpass_plan, int* modal_plan, double* tuned_ms);
+/* replace the tuner's choice (-1 keeps a phase's plan) */
+int sf_batch_select(sf_batch* b, int xpass_plan, int modal_plan);
 
 /* ---- learned preconditioner ----------------------------------------------
  * No counterpart in the reference, whose Newton-GMRES is unpreconditioned

@@ -23,6 +23,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsfhip.so")
 
 THREE_D, TWO_D, TWO_AND_A_HALF = 0, 1, 2
+BATCH_GROUPED, BATCH_MIXED, BATCH_MIXED8 = 0, 1, 2
+BATCH_PLANS = {0: "grouped", 1: "mixed", 2: "mixed8"}
 NEUMANN, DIRICHLET = 0, 1
 NODAL, MODAL, ONE_D = 0, 1, 2
 STATE_IDS = {"u1": 0, "u2": 1, "u3": 2, "b": 3, "p": 4, "U1": 5, "U2": 6, "U3": 7, "B": 8,
@@ -61,7 +63,7 @@ EXPORTED = [
     "sf_energy_f64", "sf_stage", "sf_remove_divergence", "sf_filter_all", "sf_populate_nodal",
     "sf_energy", "sf_cfl", "sf_cfl_linear", "sf_cfl_linear_noalloc", "sf_refresh_nodal", "sf_get_background", "sf_synchronize", "sf_time_steps", "sf_time_steps_detailed",
     "sf_batch_create", "sf_batch_destroy", "sf_batch_last_error", "sf_batch_step", "sf_batch_time_steps",
-    "sf_batch_info",
+    "sf_batch_info", "sf_batch_plan", "sf_batch_select",
     "sf_net_create", "sf_net_destroy", "sf_net_last_error", "sf_net_apply", "sf_net_add_pair", "sf_net_clear_pairs",
     "sf_net_samples", "sf_net_train", "sf_net_param_count", "sf_net_get_params", "sf_net_set_params",
     "sf_net_geometry", "sf_net_forward_raw", "sf_net_grad_raw", "sf_net_set_dilations",
@@ -141,6 +143,8 @@ def lib():
         "sf_batch_time_steps": [vp, i, dp],
         "sf_batch_info": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                           ctypes.POINTER(ctypes.c_int)],
+        "sf_batch_plan": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), dp],
+        "sf_batch_select": [vp, i, i],
         "sf_net_create": [vp, i, i, ctypes.c_ulonglong, vpp],
         "sf_net_destroy": [vp],
         "sf_net_apply": [vp, vpp, vpp],
@@ -546,7 +550,18 @@ class Batch:
     def info(self):
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self._chk(lib().sf_batch_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return {"launches_per_step": a.value, "xpass_groups": b.value, "modal_groups": c.value}
+        x, m = ctypes.c_int(), ctypes.c_int()
+        t = np.zeros(5)
+        self._chk(lib().sf_batch_plan(self.h, ctypes.byref(x), ctypes.byref(m), _p(t)))
+        return {"launches_per_step": a.value, "xpass_groups": b.value, "modal_groups": c.value,
+                "xpass_plan": BATCH_PLANS[x.value], "modal_plan": BATCH_PLANS[m.value],
+                "tuned_ms_per_stage": dict(zip(["xpass_grouped", "xpass_mixed", "modal_grouped", "modal_mixed4",
+                                                "modal_mixed8"], t.tolist()))}
+
+    def select(self, xpass=-1, modal=-1):
+        """Override the tuner's plan of a phase (BATCH_GROUPED, BATCH_MIXED,
+        BATCH_MIXED8 for the modal phase; -1 keeps it)."""
+        self._chk(lib().sf_batch_select(self.h, int(xpass), int(modal)))
 
     def close(self):
         if getattr(self, "h", None):

@@ -387,6 +387,7 @@ struct Ctx {
                 mark(SF_K_YINV);
             }
             XArgs<T> x{};
+            x.N1 = N1;
             x.N3 = N3;
             x.N3p = N3p;
             x.A1 = A1;
@@ -430,6 +431,7 @@ struct Ctx {
     XArgs<T> xargs_2d(bool linear, bool from_state = false) {
         auto src = [&](int i) -> const V* { return from_state ? mf(SF_U1 + i) : this->src(i); };
         XArgs<T> x{};
+        x.N1 = N1;
         x.N3 = N3;
         x.N3p = N3p;
         x.A1 = A1;
@@ -911,30 +913,43 @@ struct Ctx {
 
 // ---------------------------------------------------------------------------
 // Batched ensemble: many independent 2D runs (each its own context, grid and
-// parameters) advanced together.  Runs with the same N1 share one x-pass
-// launch and runs with the same rows-per-lane share one modal launch; both
-// kernels loop over the work items of all their runs (grid-stride, so a fixed
-// number of resident blocks drains every run).  One step is 3 x (x-pass
-// groups + modal groups) launches, captured once in a HIP graph and replayed.
-// The levels per x-pass block and the number of resident blocks of each
-// x-pass group are picked by timing the candidates when the batch is built.
+// parameters) advanced together.  Each phase of a stage has two plans:
+//   x-pass: one launch per distinct N1 (k_xpass_batch), or one launch for all
+//           runs (k_xpass_mixed, each work item dispatching on its run's N1);
+//   modal:  one launch per distinct rows-per-lane M (k_modal_batch), or one
+//           launch for all runs with 4 or 8 modes per block (k_modal_mixed).
+// Both kernels loop over the work items of all their runs (grid-stride, so a
+// fixed number of resident blocks drains every run).  The autotuner times
+// every candidate when the batch is built -- the x-pass plans with their
+// levels per block and resident-block counts on the runs themselves (the
+// x-pass only rewrites the nonlinear terms), the modal plans on scratch
+// copies of the state -- and keeps the fastest of each phase.  Every plan does
+// the same arithmetic per mode and per level, so the choice changes the time
+// only.  One step of all runs is captured in a HIP graph and replayed.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Batch {
+    using V = cx<T>;
     struct Group {
-        int key = 0;  // N1 (x-pass) or rows per lane M (modal)
+        int key = 0;  // N1 (x-pass), rows per lane M (grouped modal), modes per block (mixed modal)
         std::vector<int> members;
         int total = 0;
         int grid = 1;
         int zch = 2;
+        int span = 1;  // mixed launches: the largest N1 (x-pass) or rows per lane (modal) of the members
         void* args = nullptr;
         int* start = nullptr;
     };
     std::vector<Ctx<T>*> runs;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<Group> xg;
+    std::vector<Group> xg;  // x-pass, one group per N1
+    Group xmix;             // x-pass, all runs
     std::vector<Group> mg[3];
+    Group mmix[3];
+    int xplan = SF_BATCH_GROUPED;
+    int mplan = SF_BATCH_GROUPED;
+    std::vector<double> tuned_ms;  // x grouped, x mixed, modal grouped, modal mixed 4, modal mixed 8
     std::vector<double> dts;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -944,8 +959,11 @@ struct Batch {
     ~Batch() {
         release_graph();
         for (auto& g : xg) free_group(g);
-        for (auto& v : mg)
-            for (auto& g : v) free_group(g);
+        free_group(xmix);
+        for (int k = 0; k < 3; ++k) {
+            for (auto& g : mg[k]) free_group(g);
+            free_group(mmix[k]);
+        }
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -973,11 +991,12 @@ struct Batch {
         SF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         SF_HIP(hipEventCreate(&ev0));
         SF_HIP(hipEventCreate(&ev1));
-        // groups
         std::map<int, std::vector<int>> byN1, byM;
+        std::vector<int> all;
         for (int i = 0; i < int(runs.size()); ++i) {
             byN1[runs[i]->N1].push_back(i);
             byM[(runs[i]->N3 + 63) / 64].push_back(i);
+            all.push_back(i);
         }
         for (auto& kv : byN1) {
             Group g;
@@ -985,13 +1004,24 @@ struct Batch {
             g.members = kv.second;
             xg.push_back(g);
         }
-        for (int k = 0; k < 3; ++k)
+        xmix.members = all;
+        int nmax = 0, mmax = 0;
+        for (auto* c : runs) {
+            nmax = std::max(nmax, c->N1);
+            mmax = std::max(mmax, (c->N3 + 63) / 64);
+        }
+        xmix.span = nmax;
+        for (int k = 0; k < 3; ++k) {
             for (auto& kv : byM) {
                 Group g;
                 g.key = kv.first;
                 g.members = kv.second;
                 mg[k].push_back(g);
             }
+            mmix[k].members = all;
+            mmix[k].key = 4;
+            mmix[k].span = mmax;
+        }
         sync_runs();
         tune();
         build();
@@ -1028,71 +1058,125 @@ struct Batch {
         upload_group(g, a, st);
     }
 
-    void build() {
-        release_graph();
-        for (auto& g : xg) build_xgroup(g, g.zch, g.grid);
-        for (int k = 0; k < 3; ++k)
-            for (auto& g : mg[k]) {
-                std::vector<ModalArgs<T>> a;
-                std::vector<int> st;
-                int total = 0;
-                for (int r : g.members) {
-                    Ctx<T>* c = runs[r];
-                    // the explicit terms ping-pong as in Ctx::stage: stage 0
-                    // writes rbuf[1-rcur], stage 1 reads it and writes rbuf[rcur],
-                    // stage 2 reads rbuf[rcur]; after a step rcur is unchanged
-                    const int rc = (k == 1) ? 1 - c->rcur : c->rcur;
-                    a.push_back(c->stage_modal_args(k, k < 2, rc));
-                    st.push_back(total);
-                    const int mb = modal_modes_per_block(g.key, int(sizeof(T)));
-                    total += ((c->A1 + mb - 1) / mb) * c->A2;
-                }
-                st.push_back(total);
-                g.total = total;
-                g.grid = std::max(1, std::min(total, 8 * ncu));
-                upload_group(g, a, st);
-            }
-        dts.clear();
-        for (auto* c : runs) dts.push_back(c->dt);
-        launches_per_step = 3 * int(xg.size() + mg[0].size());
+    // modal group: arguments of run r from args(r), mb modes per block for
+    // every member (mixed) or the kernel's own modes per block for M = key
+    template <typename Fn>
+    void build_mgroup(Group& g, bool mixed, Fn args) {
+        std::vector<ModalArgs<T>> a;
+        std::vector<int> st;
+        int total = 0;
+        for (int r : g.members) {
+            Ctx<T>* c = runs[r];
+            a.push_back(args(r));
+            st.push_back(total);
+            const int mb = mixed ? g.key : modal_modes_per_block(g.key, int(sizeof(T)));
+            total += ((c->A1 + mb - 1) / mb) * c->A2;
+        }
+        st.push_back(total);
+        g.total = total;
+        g.grid = std::max(1, std::min(total, 8 * ncu));
+        upload_group(g, a, st);
     }
 
-    void launch_stage(int k) {
+    // the explicit terms ping-pong as in Ctx::stage: stage 0 writes
+    // rbuf[1-rcur], stage 1 reads it and writes rbuf[rcur], stage 2 reads
+    // rbuf[rcur]; after a step rcur is unchanged
+    ModalArgs<T> stage_args(int r, int k) {
+        Ctx<T>* c = runs[r];
+        const int rc = (k == 1) ? 1 - c->rcur : c->rcur;
+        return c->stage_modal_args(k, k < 2, rc);
+    }
+
+    void build() {
+        release_graph();
+        if (xplan == SF_BATCH_MIXED) {
+            build_xgroup(xmix, xmix.zch, xmix.grid);
+        } else {
+            for (auto& g : xg) build_xgroup(g, g.zch, g.grid);
+        }
+        for (int k = 0; k < 3; ++k) {
+            if (mplan == SF_BATCH_GROUPED) {
+                for (auto& g : mg[k]) build_mgroup(g, false, [&](int r) { return stage_args(r, k); });
+            } else {
+                mmix[k].key = (mplan == SF_BATCH_MIXED8) ? 8 : 4;
+                build_mgroup(mmix[k], true, [&](int r) { return stage_args(r, k); });
+            }
+        }
+        dts.clear();
+        for (auto* c : runs) dts.push_back(c->dt);
+        launches_per_step = 3 * ((xplan == SF_BATCH_MIXED ? 1 : int(xg.size())) +
+                                 (mplan == SF_BATCH_GROUPED ? int(mg[0].size()) : 1));
+    }
+
+    void launch_x(bool mixed) {
+        if (mixed) {
+            const hipError_t e = launch_xpass_mixed<T>(static_cast<const XArgs<T>*>(xmix.args), xmix.start,
+                                                       int(xmix.members.size()), xmix.total, xmix.span, xmix.grid,
+                                                       stream);
+            if (e != hipSuccess) throw Error(SF_ERR_HIP, std::string("batched xpass: ") + hipGetErrorString(e));
+            return;
+        }
         for (auto& g : xg) {
             const hipError_t e = launch_xpass_batch<T>(static_cast<const XArgs<T>*>(g.args), g.start,
                                                        int(g.members.size()), g.total, g.key, g.grid, stream);
             if (e != hipSuccess) throw Error(SF_ERR_HIP, std::string("batched xpass: ") + hipGetErrorString(e));
         }
-        for (auto& g : mg[k]) {
+    }
+
+    void launch_m(int plan, std::vector<Group>& groups, Group& mix) {
+        if (plan != SF_BATCH_GROUPED) {
+            const hipError_t e =
+                launch_modal_mixed<T>(static_cast<const ModalArgs<T>*>(mix.args), mix.start, int(mix.members.size()),
+                                      mix.total, mix.key, mix.span, mix.grid, stream);
+            if (e != hipSuccess) throw Error(SF_ERR_HIP, std::string("batched modal: ") + hipGetErrorString(e));
+            return;
+        }
+        for (auto& g : groups) {
             const hipError_t e = launch_modal_batch<T>(static_cast<const ModalArgs<T>*>(g.args), g.start,
                                                        int(g.members.size()), g.total, g.key, g.grid, stream);
             if (e != hipSuccess) throw Error(SF_ERR_HIP, std::string("batched modal: ") + hipGetErrorString(e));
         }
     }
 
-    // levels per block and resident blocks of each x-pass group, by timing
-    // (the x-pass only reads the state and rewrites the nonlinear terms)
+    void launch_stage(int k) {
+        launch_x(xplan == SF_BATCH_MIXED);
+        launch_m(mplan, mg[k], mmix[k]);
+    }
+
+    // GPU time of `reps` calls of fn after one untimed call
+    template <typename Fn>
+    double time_ms(Fn fn, int reps = 3) {
+        fn();
+        SF_HIP(hipEventRecord(ev0, stream));
+        for (int i = 0; i < reps; ++i) fn();
+        SF_HIP(hipEventRecord(ev1, stream));
+        SF_HIP(hipEventSynchronize(ev1));
+        float ms = 0;
+        SF_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        return double(ms) / reps;
+    }
+
     void tune() {
-        for (auto& g : xg) {
+        tuned_ms.assign(5, 0.0);
+        // x-pass: levels per block and resident blocks of each N1 group, and of
+        // the single launch over all runs
+        auto best_x = [&](Group& g, bool mixed) {
             double best = 1e30;
             int bz = 2, bgrid = 8 * ncu;
             for (int zch : {2, 4, 8, 16}) {
                 for (int mult : {4, 8, 16, 0}) {
                     const int grid = mult ? mult * ncu : (1 << 30);
                     build_xgroup(g, zch, grid);
-                    auto once = [&]() {
-                        const hipError_t e = launch_xpass_batch<T>(static_cast<const XArgs<T>*>(g.args), g.start,
-                                                                   int(g.members.size()), g.total, g.key, g.grid,
-                                                                   stream);
+                    const double ms = time_ms([&]() {
+                        hipError_t e;
+                        if (mixed)
+                            e = launch_xpass_mixed<T>(static_cast<const XArgs<T>*>(g.args), g.start,
+                                                      int(g.members.size()), g.total, g.span, g.grid, stream);
+                        else
+                            e = launch_xpass_batch<T>(static_cast<const XArgs<T>*>(g.args), g.start,
+                                                      int(g.members.size()), g.total, g.key, g.grid, stream);
                         if (e != hipSuccess) throw Error(SF_ERR_HIP, "batched xpass (tuning)");
-                    };
-                    once();
-                    SF_HIP(hipEventRecord(ev0, stream));
-                    for (int i = 0; i < 3; ++i) once();
-                    SF_HIP(hipEventRecord(ev1, stream));
-                    SF_HIP(hipEventSynchronize(ev1));
-                    float ms = 0;
-                    SF_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+                    });
                     if (ms < best) {
                         best = ms;
                         bz = zch;
@@ -1102,7 +1186,80 @@ struct Batch {
             }
             g.zch = bz;
             g.grid = bgrid;
+            return best;
+        };
+        for (auto& g : xg) tuned_ms[0] += best_x(g, false);
+        tuned_ms[1] = best_x(xmix, true);
+        xplan = (tuned_ms[1] < tuned_ms[0]) ? SF_BATCH_MIXED : SF_BATCH_GROUPED;
+
+        // modal: stage-0 launches on scratch copies of each run's state (the
+        // fields the stage rewrites and its explicit-term outputs)
+        std::vector<V*> scratch(runs.size(), nullptr);
+        auto release = [&]() {
+            for (V*& p : scratch)
+                if (p) (void)hipFree(p);
+        };
+        try {
+            std::vector<ModalArgs<T>> sa(runs.size());
+            for (size_t r = 0; r < runs.size(); ++r) {
+                Ctx<T>* c = runs[r];
+                const size_t n = c->nmodal;
+                SF_HIP(hipMalloc(reinterpret_cast<void**>(&scratch[r]), 9 * n * sizeof(V)));
+                V* w = scratch[r];
+                ModalArgs<T> m = stage_args(int(r), 0);
+                V** rw[5] = {&m.u1, &m.u2, &m.u3, &m.b, &m.p};
+                for (int i = 0; i < 5; ++i)
+                    if (*rw[i]) {
+                        SF_HIP(hipMemcpyAsync(w + i * n, *rw[i], n * sizeof(V), hipMemcpyDeviceToDevice, stream));
+                        *rw[i] = w + i * n;
+                    }
+                V** wo[4] = {&m.rn1, &m.rn2, &m.rn3, &m.rnB};
+                for (int i = 0; i < 4; ++i)
+                    if (*wo[i]) *wo[i] = w + (5 + i) * n;
+                m.ubar = nullptr;
+                sa[r] = m;
+            }
+            auto sargs = [&](int r) { return sa[r]; };
+            std::vector<Group> tg = mg[0];
+            for (auto& g : tg) {
+                g.args = nullptr;
+                g.start = nullptr;
+                build_mgroup(g, false, sargs);
+            }
+            tuned_ms[2] = time_ms([&]() { launch_m(SF_BATCH_GROUPED, tg, mmix[0]); });
+            for (auto& g : tg) free_group(g);
+            Group tm = mmix[0];
+            tm.args = nullptr;
+            tm.start = nullptr;
+            for (int q = 0; q < 2; ++q) {
+                tm.key = q ? 8 : 4;
+                build_mgroup(tm, true, sargs);
+                tuned_ms[3 + q] = time_ms([&]() { launch_m(q ? SF_BATCH_MIXED8 : SF_BATCH_MIXED, tg, tm); });
+            }
+            free_group(tm);
+            SF_HIP(hipStreamSynchronize(stream));
+        } catch (...) {
+            release();
+            throw;
         }
+        release();
+        mplan = SF_BATCH_GROUPED;
+        double best = tuned_ms[2];
+        if (tuned_ms[3] < best) {
+            best = tuned_ms[3];
+            mplan = SF_BATCH_MIXED;
+        }
+        if (tuned_ms[4] < best) mplan = SF_BATCH_MIXED8;
+    }
+
+    // a plan chosen by the caller instead of the tuner (-1 keeps a phase's)
+    void select(int x, int m) {
+        SF_REQUIRE(x == -1 || x == SF_BATCH_GROUPED || x == SF_BATCH_MIXED, SF_ERR_ARG, "bad x-pass plan");
+        SF_REQUIRE(m == -1 || m == SF_BATCH_GROUPED || m == SF_BATCH_MIXED || m == SF_BATCH_MIXED8, SF_ERR_ARG,
+                   "bad modal plan");
+        if (x != -1) xplan = x;
+        if (m != -1) mplan = m;
+        build();
     }
 
     void capture() {
@@ -1904,6 +2061,21 @@ int sf_batch_info(const sf_batch* b, int* launches_per_step, int* xpass_groups, 
         if (xpass_groups) *xpass_groups = int(B.xg.size());
         if (modal_groups) *modal_groups = int(B.mg[0].size());
     });
+}
+
+int sf_batch_plan(const sf_batch* b, int* xpass_plan, int* modal_plan, double* tuned_ms) {
+    SF_NULLCHK(b);
+    return batch_call(const_cast<sf_batch*>(b), [&](auto& B) {
+        if (xpass_plan) *xpass_plan = B.xplan;
+        if (modal_plan) *modal_plan = B.mplan;
+        if (tuned_ms)
+            for (int i = 0; i < 5; ++i) tuned_ms[i] = B.tuned_ms[i];
+    });
+}
+
+int sf_batch_select(sf_batch* b, int xpass_plan, int modal_plan) {
+    SF_NULLCHK(b);
+    return batch_call(b, [&](auto& B) { B.select(xpass_plan, modal_plan); });
 }
 
 }  // extern "C"

@@ -34,7 +34,7 @@ __host__ __device__ inline size_t sidx(int row, int z, int j1, int N3, int A1) {
 // Fused inverse-x / products / forward-x kernel.
 template <typename T>
 struct XArgs {
-    int N3, N3p, A1, zch;                // zch: z levels per block
+    int N1, N3, N3p, A1, zch;            // zch: z levels per block
     const cx<T>* in[4];                  // u1, u2, u3, b spectra
     cx<T>* out[8];                       // 3D: A1,A2,A3,AB,B1,B2,B3,BB ; 2D: Nl1,-,Nl3,NlB
     const T* zc;
@@ -74,13 +74,23 @@ hipError_t launch_modal(const ModalArgs<T>& a, hipStream_t s);
 template <typename T>
 hipError_t launch_xpass(const XArgs<T>& a, int N1, int N2, bool threeD, hipStream_t s);
 // many independent 2D runs per launch: args/start live in device memory,
-// start[r] = first work item of run r (x-pass: z-blocks; modal: groups of
-// four modes), total = start[nruns]; grid-stride loop over `grid` blocks
+// start[r] = first work item of run r (x-pass: z-blocks; modal: tiles of
+// modes), total = start[nruns]; grid-stride loop over `grid` blocks.  The
+// _batch launches take runs of one N1 (x-pass) or one rows-per-lane M
+// (modal); the _mixed launches take runs of any N1 / N3, the modal one with
+// mb (4 or 8) modes per block for every run.
 template <typename T>
 hipError_t launch_modal_batch(const ModalArgs<T>* args, const int* start, int nruns, int total, int M, int grid,
                               hipStream_t s);
 template <typename T>
 hipError_t launch_xpass_batch(const XArgs<T>* args, const int* start, int nruns, int total, int N1, int grid,
+                              hipStream_t s);
+// (mmax: the largest rows per lane of the runs; nmax: the largest N1)
+template <typename T>
+hipError_t launch_modal_mixed(const ModalArgs<T>* args, const int* start, int nruns, int total, int mb, int mmax,
+                              int grid, hipStream_t s);
+template <typename T>
+hipError_t launch_xpass_mixed(const XArgs<T>* args, const int* start, int nruns, int total, int nmax, int grid,
                               hipStream_t s);
 template <typename T>
 hipError_t launch_yinv(const YArgs<T>& a, int N2, hipStream_t s);

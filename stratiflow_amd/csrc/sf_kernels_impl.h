@@ -1,4 +1,6 @@
-// sf_kernels.hip -- gfx950 kernels for the IMEX-RK timestep.
+// sf_kernels_impl.h -- gfx950 kernels for the IMEX-RK timestep.  The .hip
+// files sf_kernels_{f32,f64}.hip and sf_batch_{f32,f64}.hip instantiate them
+// (four translation units, compiled in parallel).
 //
 // Per RK stage the reference runs ExplicitRK, BuildRHS, FinishRHS,
 // CrankNicolson, RemoveDivergence, FilterAll and PopulateNodalVariables
@@ -16,6 +18,7 @@
 //            matvec and implicit solves, divergence, Poisson solve,
 //            projection and pressure update, all on one wavefront.
 // Modal fields hold only the active (dealiased) modes, [j2a][j1][z].
+#pragma once
 #include "sf_fft.h"
 #include "sf_kernels.h"
 #include "sf_tri.h"
@@ -118,6 +121,9 @@ __device__ __forceinline__ void build_rows(int kind, const ModalArgs<T>& A, cons
     }
 }
 
+// largest rows per lane of the modal kernel (N3 <= 64 * MODAL_MAX_M)
+constexpr int MODAL_MAX_M = 8;
+
 // modes per block of the modal kernel: 4 consecutive j1, one wavefront each
 // (two such blocks share a CU at the register use of M = 6)
 template <typename T, int M>
@@ -131,10 +137,9 @@ constexpr int modal_mb() {
 // holds the column of mode j10+w with lane l owning rows l*M .. l*M+M-1.
 // Waves past A1 run on zeros and store nothing; they keep the block's barriers.
 template <typename T, int M, int MB>
-__device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, const int j10) {
+__device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, const int j10, cx<T>* const tile) {
     using V = cx<T>;
-    constexpr int TP = MB + 1;  // tile row pitch
-    __shared__ V tile[64 * M * TP];
+    constexpr int TP = MB + 1;  // tile row pitch (tile: 64*M*TP values of LDS)
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int A1 = A.A1;
@@ -440,9 +445,11 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 
 template <typename T, int M>
 __global__ __launch_bounds__((modal_mb<T, M>() * 64)) void k_modal(ModalArgs<T> A) {
+    constexpr int MB = modal_mb<T, M>();
+    __shared__ cx<T> tile[64 * M * (MB + 1)];
     const int nt = gridDim.x;  // j1 tiles per ja
     const int t = xcd_tile(blockIdx.x + nt * blockIdx.y, nt * gridDim.y);
-    modal_body<T, M, modal_mb<T, M>()>(A, t / nt, (t % nt) * modal_mb<T, M>());
+    modal_body<T, M, MB>(A, t / nt, (t % nt) * MB, tile);
 }
 
 // index r of the run owning work item `item`: start[r] <= item < start[r+1]
@@ -464,15 +471,67 @@ __global__ __launch_bounds__((modal_mb<T, M>() * 64)) void k_modal_batch(const M
                                                                       const int* __restrict__ start, int nruns,
                                                                       int total) {
     constexpr int MB = modal_mb<T, M>();
+    __shared__ cx<T> tile[64 * M * (MB + 1)];
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const int r = find_run(start, nruns, item);
         const ModalArgs<T>& a = args[r];
         const int nt = (a.A1 + MB - 1) / MB;
         const int local = item - start[r];
-        modal_body<T, M, MB>(a, local / nt, (local % nt) * MB);
+        modal_body<T, M, MB>(a, local / nt, (local % nt) * MB, tile);
     }
 }
 
+// All runs in one launch whatever their N3: each work item dispatches on its
+// run's rows per lane M; the block has MB waves for every M and one LDS tile
+// sized for the largest M.  Resources (registers, LDS) are those of the most
+// demanding case compiled in.
+template <typename T, int MB, int MMAX>
+__global__ __launch_bounds__(MB * 64) void k_modal_mixed(const ModalArgs<T>* __restrict__ args,
+                                                        const int* __restrict__ start, int nruns, int total) {
+    __shared__ cx<T> tile[64 * MMAX * (MB + 1)];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int r = find_run(start, nruns, item);
+        const ModalArgs<T>& a = args[r];
+        const int nt = (a.A1 + MB - 1) / MB;
+        const int local = item - start[r];
+        const int ja = local / nt, j10 = (local % nt) * MB;
+        switch ((a.N3 + 63) / 64) {
+#define SF_M(m)                                                       \
+    case m:                                                           \
+        if constexpr (m <= MMAX) modal_body<T, m, MB>(a, ja, j10, tile); \
+        break;
+            SF_M(1) SF_M(2) SF_M(3) SF_M(4) SF_M(5) SF_M(6) SF_M(7) SF_M(8)
+#undef SF_M
+            default:
+                break;
+        }
+    }
+}
+
+// the variant compiled for rows per lane up to 3, 6 or 8 that covers mmax
+template <typename T, int MB>
+static void launch_modal_mixed_mb(const ModalArgs<T>* args, const int* start, int nruns, int total, int mmax, int grid,
+                                  hipStream_t s) {
+    if (mmax <= 3)
+        hipLaunchKernelGGL((k_modal_mixed<T, MB, 3>), dim3(grid), dim3(MB * 64), 0, s, args, start, nruns, total);
+    else if (mmax <= 6)
+        hipLaunchKernelGGL((k_modal_mixed<T, MB, 6>), dim3(grid), dim3(MB * 64), 0, s, args, start, nruns, total);
+    else
+        hipLaunchKernelGGL((k_modal_mixed<T, MB, 8>), dim3(grid), dim3(MB * 64), 0, s, args, start, nruns, total);
+}
+
+template <typename T>
+hipError_t launch_modal_mixed(const ModalArgs<T>* args, const int* start, int nruns, int total, int mb, int mmax,
+                              int grid, hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    if (mmax < 1 || mmax > MODAL_MAX_M) return hipErrorInvalidValue;
+    if (mb == 4) launch_modal_mixed_mb<T, 4>(args, start, nruns, total, mmax, grid, s);
+    else if (mb == 8) launch_modal_mixed_mb<T, 8>(args, start, nruns, total, mmax, grid, s);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+#ifdef SF_KERNELS_HOST_DEFS
 int modal_modes_per_block(int M, int precision) {
     switch (M) {
 #define SF_M(m) \
@@ -484,6 +543,7 @@ int modal_modes_per_block(int M, int precision) {
             return 4;
     }
 }
+#endif
 
 template <typename T>
 hipError_t launch_modal_batch(const ModalArgs<T>* args, const int* start, int nruns, int total, int M, int grid,
@@ -521,7 +581,9 @@ hipError_t launch_modal(const ModalArgs<T>& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-int max_rows_per_lane() { return 8; }
+#ifdef SF_KERNELS_HOST_DEFS
+int max_rows_per_lane() { return MODAL_MAX_M; }
+#endif
 
 // ---------------------------------------------------------------------------
 // k_xpass: inverse x FFT, products, forward x FFT
@@ -537,6 +599,7 @@ int max_rows_per_lane() { return 8; }
 // ---------------------------------------------------------------------------
 #define SF_SIZES(X) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
 
+#ifdef SF_KERNELS_HOST_DEFS
 bool tz_supported(int n) {
     switch (n) {
 #define SF_CASE(n) case n:
@@ -547,6 +610,7 @@ bool tz_supported(int n) {
             return false;
     }
 }
+#endif
 
 // line pitch of the x-pass: the smallest that holds the FFT's padded pass-1
 // table, rounded to 8 mod 32 complex values (keeps four 3D blocks per CU)
@@ -570,8 +634,15 @@ constexpr int xpass_bs() {
     return BS;
 }
 
+// LDS values of one x-pass block: NP product lines and a ring of 3 x NI
+// input lines at the line pitch x_ldp
+template <int N1, bool THREED, bool LIN>
+constexpr int xpass_lds() {
+    return ((THREED ? 6 : 3) + 3 * (LIN ? 4 : 2)) * x_ldp<N1>();
+}
+
 template <typename T, int N1, bool THREED, bool LIN>
-__device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, const int n2) {
+__device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, const int n2, cx<T>* const lds) {
     using V = cx<T>;
     constexpr int LDP = x_ldp<N1>();
     constexpr int NI = LIN ? 4 : 2;        // input pair lines per level
@@ -580,8 +651,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
     constexpr int AMAX = N1 / 3;           // A1 <= N1/3
     constexpr int XB = xpass_bs<T, THREED, LIN>();
     constexpr int FI = (NI * AMAX + XB - 1) / XB;
-    // [NP product lines][ring slot 0: NI lines][slot 1][slot 2]
-    __shared__ V lds[(NP + 3 * NI) * LDP];
+    // lds: [NP product lines][ring slot 0: NI lines][slot 1][slot 2]
     V* const prd = lds;
     V* const ring = lds + NP * LDP;
 
@@ -802,7 +872,8 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
 template <typename T, int N1, bool THREED, bool LIN>
 __global__ __launch_bounds__((xpass_bs<T, THREED, LIN>())) __attribute__((amdgpu_waves_per_eu(xpass_waves<T, THREED, LIN>()))) void k_xpass(
     XArgs<T> A) {
-    xpass_body<T, N1, THREED, LIN>(A, blockIdx.x, blockIdx.y);
+    __shared__ cx<T> lds[xpass_lds<N1, THREED, LIN>()];
+    xpass_body<T, N1, THREED, LIN>(A, blockIdx.x, blockIdx.y, lds);
 }
 
 // 2D runs with the same N1 in one launch (grid-stride over the z-blocks of
@@ -811,10 +882,55 @@ template <typename T, int N1>
 __global__ __launch_bounds__(BS) void k_xpass_batch(const XArgs<T>* __restrict__ args, const int* __restrict__ start,
                                                     int nruns, int total) {
     static_assert(xpass_bs<T, false, false>() == BS, "the batched 2D x-pass runs blocks of BS threads");
+    __shared__ cx<T> lds[xpass_lds<N1, false, false>()];
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const int r = find_run(start, nruns, item);
-        xpass_body<T, N1, false, false>(args[r], item - start[r], 0);
+        xpass_body<T, N1, false, false>(args[r], item - start[r], 0, lds);
     }
+}
+
+// 2D runs of every N1 up to NMAX in one launch: each work item dispatches on
+// its run's N1 and all cases share one LDS allocation sized for the largest.
+template <int NMAX>
+constexpr int xpass_mixed_lds() {
+    int m = 0;
+#define SF_CASE(n) \
+    if (n <= NMAX && xpass_lds<n, false, false>() > m) m = xpass_lds<n, false, false>();
+    SF_SIZES(SF_CASE)
+#undef SF_CASE
+    return m;
+}
+
+template <typename T, int NMAX>
+__global__ __launch_bounds__(BS) void k_xpass_mixed(const XArgs<T>* __restrict__ args, const int* __restrict__ start,
+                                                    int nruns, int total) {
+    __shared__ cx<T> lds[xpass_mixed_lds<NMAX>()];
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const int r = find_run(start, nruns, item);
+        const XArgs<T>& a = args[r];
+        switch (a.N1) {
+#define SF_CASE(n)                                                                    \
+    case n:                                                                           \
+        if constexpr (n <= NMAX) xpass_body<T, n, false, false>(a, item - start[r], 0, lds); \
+        break;
+            SF_SIZES(SF_CASE)
+#undef SF_CASE
+            default:
+                break;
+        }
+    }
+}
+
+// the variant compiled for N1 up to 128, 256 or 512 that covers nmax
+template <typename T>
+hipError_t launch_xpass_mixed(const XArgs<T>* args, const int* start, int nruns, int total, int nmax, int grid,
+                              hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    if (nmax <= 128) hipLaunchKernelGGL((k_xpass_mixed<T, 128>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total);
+    else if (nmax <= 256) hipLaunchKernelGGL((k_xpass_mixed<T, 256>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total);
+    else if (nmax <= 512) hipLaunchKernelGGL((k_xpass_mixed<T, 512>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -835,7 +951,7 @@ hipError_t launch_xpass_batch(const XArgs<T>* args, const int* start, int nruns,
 
 template <typename T, int N1, bool THREED, bool LIN>
 static hipError_t launch_xpass_v(const XArgs<T>& a, int N2, hipStream_t s) {
-    constexpr int lds = ((THREED ? 6 : 3) + 3 * (LIN ? 4 : 2)) * x_ldp<N1>() * int(sizeof(cx<T>));
+    constexpr int lds = xpass_lds<N1, THREED, LIN>() * int(sizeof(cx<T>));
     if constexpr (lds > 160 * 1024) {
         return hipErrorInvalidValue;  // not instantiated: does not fit the LDS of one CU
     } else {
@@ -869,7 +985,7 @@ hipError_t launch_xpass(const XArgs<T>& a, int N1, int N2, bool threeD, hipStrea
 // many blocks of this kernel the GPU keeps resident.
 template <typename T, int N1, bool THREED, bool LIN>
 static int xpass_resident(int ncu) {
-    constexpr int lds = ((THREED ? 6 : 3) + 3 * (LIN ? 4 : 2)) * x_ldp<N1>() * int(sizeof(cx<T>));
+    constexpr int lds = xpass_lds<N1, THREED, LIN>() * int(sizeof(cx<T>));
     if constexpr (lds > 160 * 1024) {
         return 0;
     } else {
@@ -880,23 +996,28 @@ static int xpass_resident(int ncu) {
     }
 }
 
-int xpass_levels(int N1, int N2, int N3, bool threeD, int precision) {
-    int dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    int slots = ncu;
+// resident x-pass blocks on ncu CUs (each precision's translation unit
+// instantiates its own)
+template <typename T>
+int xpass_slots(int N1, bool threeD, int ncu) {
     switch (N1) {
-#define SF_CASE(n)                                                                                             \
-    case n:                                                                                                    \
-        slots = precision == 4 ? (threeD ? xpass_resident<float, n, true, false>(ncu)                          \
-                                         : xpass_resident<float, n, false, false>(ncu))                        \
-                               : (threeD ? xpass_resident<double, n, true, false>(ncu)                         \
-                                         : xpass_resident<double, n, false, false>(ncu));                      \
-        break;
+#define SF_CASE(n) \
+    case n:        \
+        return threeD ? xpass_resident<T, n, true, false>(ncu) : xpass_resident<T, n, false, false>(ncu);
         SF_SIZES(SF_CASE)
 #undef SF_CASE
         default:
-            break;
+            return ncu;
     }
+}
+
+#ifdef SF_KERNELS_HOST_DEFS
+extern template int xpass_slots<double>(int, bool, int);  // in the FP64 translation unit
+
+int xpass_levels(int N1, int N2, int N3, bool threeD, int precision) {
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    int slots = precision == 4 ? xpass_slots<float>(N1, threeD, ncu) : xpass_slots<double>(N1, threeD, ncu);
     if (slots < 1) slots = ncu;
     const int rows = threeD ? N2 : 1;
     long best = -1;
@@ -912,7 +1033,9 @@ int xpass_levels(int N1, int N2, int N3, bool threeD, int precision) {
     }
     return pick;
 }
+#endif
 
+#ifdef SF_KERNELS_HOST_DEFS
 void fft_twiddles(int n, double* re, double* im) {
     switch (n) {
 #define SF_CASE(n)                           \
@@ -925,6 +1048,7 @@ void fft_twiddles(int n, double* re, double* im) {
             return;
     }
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // y transforms (3D), one block per (z, tile of YJ consecutive j1)
@@ -1427,7 +1551,7 @@ __global__ __launch_bounds__(BS) void k_dot_partial(const cx<T>* __restrict__ a,
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(BS) void k_dot_final(const double* __restrict__ partial, int n, double* out, int slot) {
+static __global__ __launch_bounds__(BS) void k_dot_final(const double* __restrict__ partial, int n, double* out, int slot) {
     __shared__ double red[BS];
     double acc = 0;
     for (int i = threadIdx.x; i < n; i += BS) acc += partial[i];
@@ -1452,7 +1576,9 @@ hipError_t launch_dot(const cx<T>* a, const cx<T>* b, int A1, int A2, int N3, co
     return hipGetLastError();
 }
 
+#ifdef SF_KERNELS_HOST_DEFS
 int max_reduction_blocks() { return DOT_BLOCKS; }
+#endif
 
 // max |f + add[z]| over a nodal field (NodalField::Max, Field.h:771-786)
 template <typename T>
@@ -1475,7 +1601,7 @@ __global__ __launch_bounds__(BS) void k_maxabs_partial(const T* __restrict__ f, 
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(BS) void k_max_final(const double* __restrict__ partial, int n, double* out, int slot) {
+static __global__ __launch_bounds__(BS) void k_max_final(const double* __restrict__ partial, int n, double* out, int slot) {
     __shared__ double red[BS];
     double m = 0;
     for (int i = threadIdx.x; i < n; i += BS) m = fmax(m, partial[i]);
@@ -1736,25 +1862,25 @@ hipError_t launch_diffprec(const DiffPrecArgs<T>& p, hipStream_t s) {
     return hipGetLastError();
 }
 
-#define SF_INST(T)                                                                                                  \
-    template hipError_t launch_modal<T>(const ModalArgs<T>&, hipStream_t);                                           \
-    template hipError_t launch_modal_batch<T>(const ModalArgs<T>*, const int*, int, int, int, int, hipStream_t);      \
-    template hipError_t launch_xpass_batch<T>(const XArgs<T>*, const int*, int, int, int, int, hipStream_t);          \
-    template hipError_t launch_xpass<T>(const XArgs<T>&, int, int, bool, hipStream_t);                               \
-    template hipError_t launch_yinv<T>(const YArgs<T>&, int, hipStream_t);                                           \
-    template hipError_t launch_yfwd<T>(const YArgs<T>&, int, hipStream_t);                                           \
-    template hipError_t launch_x_c2r<T>(const XTArgs<T>&, int, int, hipStream_t);                                   \
-    template hipError_t launch_x_r2c<T>(const XTArgs<T>&, int, int, hipStream_t);                                   \
-    template hipError_t launch_dot<T>(const cx<T>*, const cx<T>*, int, int, int, const T*, double*, double*, int,    \
-                                      hipStream_t);                                                                  \
-    template hipError_t launch_axpby<T>(cx<T>*, T, const cx<T>*, T, size_t, hipStream_t);                          \
-    template hipError_t launch_maxabs<T>(const T*, size_t, int, const T*, double*, double*, int, hipStream_t);       \
-    template hipError_t launch_ends<T>(cx<T>*, int, int, int, int, hipStream_t);                                          \
-    template hipError_t launch_dmul<T>(cx<T>*, const cx<T>*, int, int, int, const T*, const T*, int, const T*, int, \
-                                       hipStream_t);                                                                 \
-    template hipError_t launch_diffprec<T>(const DiffPrecArgs<T>&, hipStream_t);
+#define SF_INST_CORE(T) \
+    template hipError_t launch_modal<T>(const ModalArgs<T>&, hipStream_t); \
+    template hipError_t launch_xpass<T>(const XArgs<T>&, int, int, bool, hipStream_t); \
+    template hipError_t launch_yinv<T>(const YArgs<T>&, int, hipStream_t); \
+    template hipError_t launch_yfwd<T>(const YArgs<T>&, int, hipStream_t); \
+    template hipError_t launch_x_c2r<T>(const XTArgs<T>&, int, int, hipStream_t); \
+    template hipError_t launch_x_r2c<T>(const XTArgs<T>&, int, int, hipStream_t); \
+    template hipError_t launch_dot<T>(const cx<T>*, const cx<T>*, int, int, int, const T*, double*, double*, int, hipStream_t); \
+    template hipError_t launch_axpby<T>(cx<T>*, T, const cx<T>*, T, size_t, hipStream_t); \
+    template hipError_t launch_maxabs<T>(const T*, size_t, int, const T*, double*, double*, int, hipStream_t); \
+    template hipError_t launch_ends<T>(cx<T>*, int, int, int, int, hipStream_t); \
+    template hipError_t launch_dmul<T>(cx<T>*, const cx<T>*, int, int, int, const T*, const T*, int, const T*, int, hipStream_t); \
+    template hipError_t launch_diffprec<T>(const DiffPrecArgs<T>&, hipStream_t); \
+    template int xpass_slots<T>(int, bool, int);
 
-SF_INST(float)
-SF_INST(double)
+#define SF_INST_BATCH(T) \
+    template hipError_t launch_modal_batch<T>(const ModalArgs<T>*, const int*, int, int, int, int, hipStream_t); \
+    template hipError_t launch_xpass_batch<T>(const XArgs<T>*, const int*, int, int, int, int, hipStream_t); \
+    template hipError_t launch_xpass_mixed<T>(const XArgs<T>*, const int*, int, int, int, int, hipStream_t); \
+    template hipError_t launch_modal_mixed<T>(const ModalArgs<T>*, const int*, int, int, int, int, int, hipStream_t);
 
 }  // namespace sf

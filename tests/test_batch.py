@@ -59,3 +59,36 @@ def test_batch_matches_individual_runs(prec):
     for s, t in zip(batched, twins):
         assert np.array_equal(s.get("u1"), t.get("u1"))
     b.close()
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_every_batch_plan_matches_individual_runs(prec):
+    """One launch per N1 / per vertical tile, or one launch for all runs (with
+    4 or 8 modes per modal block): the tuner's choice changes the time only."""
+    import stratiflow_amd as sf
+    batched = [_make(*r, seed=200 + i, prec=prec) for i, r in enumerate(RUNS)]
+    twins = [_make(*r, seed=200 + i, prec=prec) for i, r in enumerate(RUNS)]
+    b = sf.Batch(batched)
+    info = b.info()
+    print("tuned", info, flush=True)
+    t = info["tuned_ms_per_stage"]
+    assert all(v > 0 for v in t.values())
+    xbest = "mixed" if t["xpass_mixed"] < t["xpass_grouped"] else "grouped"
+    mbest = min(("grouped", "mixed", "mixed8"), key=lambda k: t[{"grouped": "modal_grouped", "mixed": "modal_mixed4",
+                                                                "mixed8": "modal_mixed8"}[k]])
+    assert (info["xpass_plan"], info["modal_plan"]) == (xbest, mbest)
+    plans = [(sf.BATCH_MIXED, sf.BATCH_MIXED), (sf.BATCH_GROUPED, sf.BATCH_MIXED8), (sf.BATCH_MIXED, sf.BATCH_GROUPED),
+             (sf.BATCH_GROUPED, sf.BATCH_GROUPED), (sf.BATCH_MIXED, sf.BATCH_MIXED8)]
+    for x, m in plans:
+        b.select(x, m)
+        info = b.info()
+        assert info["launches_per_step"] == 3 * ((1 if x else 4) + (1 if m else 3)), info
+        b.step(2)
+        for tw in twins:
+            tw.TimeStep(2)
+        for i, (s, tw) in enumerate(zip(batched, twins)):
+            for f in ("u1", "u3", "b", "p"):
+                a, c = s.get(f), tw.get(f)
+                assert np.isfinite(a).all()
+                assert np.array_equal(a, c), (x, m, i, f)
+    b.close()
