@@ -155,7 +155,9 @@ def ensemble(sf, ic, nruns=64, steps=500, seed=20261202):
            "batched_run_steps_per_s": nruns * steps / (ms_batch * 1e-3),
            "sequential_run_steps_per_s": nruns * steps / (ms_seq * 1e-3),
            "speedup": ms_seq / ms_batch, "launches_per_step": info["launches_per_step"],
-           "xpass_groups": info["xpass_groups"], "modal_groups": info["modal_groups"]}
+           "xpass_groups": info["xpass_groups"], "modal_groups": info["modal_groups"],
+           "xpass_plan": info["xpass_plan"], "modal_plan": info["modal_plan"],
+           "tuned_ms_per_stage": {k: round(v, 5) for k, v in info["tuned_ms_per_stage"].items()}}
     b.close()
     return out
 

@@ -268,7 +268,8 @@ int sf_time_steps_detailed(sf_ctx* ctx, int nsteps, double* total_ms, double* ke
  * together, and one step of all runs is a HIP graph replay.  Each phase of a
  * stage runs under one of two plans: one launch per distinct N1 (x-pass) or
  * per vertical tile (modal), or ONE launch for all runs (SF_BATCH_MIXED; the
- * modal kernel with 4 or 8 modes per block).  sf_batch_create times every
+ * x-pass with either FFT plan, the modal kernel with 4 or 8 modes per block).
+ * sf_batch_create times every
  * plan, with its levels per block and resident blocks, and keeps the fastest
  * (sf_batch_plan reports the choice and the timings); every plan gives the
  * same results bit for bit.  Each context keeps its own state, set up through the
@@ -284,11 +285,12 @@ int sf_batch_step(sf_batch* b, int nsteps);
 /* as sf_batch_step, returning the GPU time of the graph replays */
 int sf_batch_time_steps(sf_batch* b, int nsteps, double* total_ms);
 int sf_batch_info(const sf_batch* b, int* launches_per_step, int* xpass_groups, int* modal_groups);
-#define SF_BATCH_GROUPED 0 /* one launch per N1 (x-pass) or per vertical tile (modal) */
-#define SF_BATCH_MIXED 1   /* one launch for all runs (modal: 4 modes per block) */
-#define SF_BATCH_MIXED8 2  /* modal only: one launch, 8 modes per block */
-/* the plans in use and the tuner's GPU times per stage in ms: x-pass grouped,
- * x-pass mixed, modal grouped, modal mixed (4), modal mixed (8) */
+#define SF_BATCH_GROUPED 0   /* one launch per N1 (x-pass) or per vertical tile (modal) */
+#define SF_BATCH_MIXED 1     /* one launch for all runs (x-pass: default FFT plans; modal: 4 modes per block) */
+#define SF_BATCH_MIXED_R8 2  /* x-pass: one launch, FFT plans with first radix 8 where N1 allows (N1/8 <= 32) */
+#define SF_BATCH_MIXED8 2    /* modal: one launch, 8 modes per block */
+/* the plans in use and the tuner's GPU times per stage in ms (six values):
+ * x-pass grouped, mixed, mixed radix 8; modal grouped, mixed (4), mixed (8) */
 int sf_batch_plan(const sf_batch* b, int* xpass_plan, int* modal_plan, double* tuned_ms);
 /* replace the tuner's choice (-1 keeps a phase's plan) */
 int sf_batch_select(sf_batch* b, int xpass_plan, int modal_plan);

@@ -23,8 +23,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsfhip.so")
 
 THREE_D, TWO_D, TWO_AND_A_HALF = 0, 1, 2
-BATCH_GROUPED, BATCH_MIXED, BATCH_MIXED8 = 0, 1, 2
-BATCH_PLANS = {0: "grouped", 1: "mixed", 2: "mixed8"}
+BATCH_GROUPED, BATCH_MIXED, BATCH_MIXED_R8, BATCH_MIXED8 = 0, 1, 2, 2
+XPASS_PLANS = {0: "grouped", 1: "mixed", 2: "mixed_r8"}
+MODAL_PLANS = {0: "grouped", 1: "mixed4", 2: "mixed8"}
 NEUMANN, DIRICHLET = 0, 1
 NODAL, MODAL, ONE_D = 0, 1, 2
 STATE_IDS = {"u1": 0, "u2": 1, "u3": 2, "b": 3, "p": 4, "U1": 5, "U2": 6, "U3": 7, "B": 8,
@@ -551,16 +552,17 @@ class Batch:
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self._chk(lib().sf_batch_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         x, m = ctypes.c_int(), ctypes.c_int()
-        t = np.zeros(5)
+        t = np.zeros(6)
         self._chk(lib().sf_batch_plan(self.h, ctypes.byref(x), ctypes.byref(m), _p(t)))
         return {"launches_per_step": a.value, "xpass_groups": b.value, "modal_groups": c.value,
-                "xpass_plan": BATCH_PLANS[x.value], "modal_plan": BATCH_PLANS[m.value],
-                "tuned_ms_per_stage": dict(zip(["xpass_grouped", "xpass_mixed", "modal_grouped", "modal_mixed4",
-                                                "modal_mixed8"], t.tolist()))}
+                "xpass_plan": XPASS_PLANS[x.value], "modal_plan": MODAL_PLANS[m.value],
+                "tuned_ms_per_stage": dict(zip(["xpass_grouped", "xpass_mixed", "xpass_mixed_r8", "modal_grouped",
+                                                "modal_mixed4", "modal_mixed8"], t.tolist()))}
 
     def select(self, xpass=-1, modal=-1):
         """Override the tuner's plan of a phase (BATCH_GROUPED, BATCH_MIXED,
-        BATCH_MIXED8 for the modal phase; -1 keeps it)."""
+        BATCH_MIXED_R8 for the x-pass, BATCH_MIXED8 for the modal phase; -1
+        keeps it)."""
         self._chk(lib().sf_batch_select(self.h, int(xpass), int(modal)))
 
     def close(self):

@@ -94,6 +94,7 @@ struct Ctx {
     T* zc = nullptr;
     T *kx = nullptr, *ky = nullptr, *kx2 = nullptr, *ky2 = nullptr;
     V *tw1 = nullptr, *tw2 = nullptr;
+    V* tw1_r8 = nullptr;  // x twiddles of the plan with first radix 8 (made on first use)
     double* partial = nullptr;
     double* scal = nullptr;
     double* scal_host = nullptr;
@@ -324,6 +325,26 @@ struct Ctx {
         tw(N1, tw1);
         tw(N2, tw2);
         SF_HIP(hipStreamSynchronize(stream));
+    }
+
+    // x-direction twiddles for the FFT plan with first radix r1f (0: default)
+    const V* x_twiddles(int r1f) {
+        if (r1f == 0 || fft_first_radix(N1, r1f) == fft_first_radix(N1, 0)) return tw1;
+        SF_REQUIRE(r1f == 8, SF_ERR_ARG, "x FFT plans: first radix 0 (default) or 8");
+        if (!tw1_r8) {
+            std::vector<double> re(N1), im(N1);
+            fft_twiddles_plan(N1, 8, re.data(), im.data());
+            std::vector<V> w(N1);
+            for (int k = 0; k < N1; ++k) {
+                w[k].x = T(re[k]);
+                w[k].y = T(im[k]);
+            }
+            V* d = dalloc<V>(N1);
+            SF_HIP(hipMemcpyAsync(d, w.data(), N1 * sizeof(V), hipMemcpyHostToDevice, stream));
+            SF_HIP(hipStreamSynchronize(stream));
+            tw1_r8 = d;
+        }
+        return tw1_r8;
     }
 
     // U_ from the device into the host copy of the coefficient table
@@ -915,7 +936,8 @@ struct Ctx {
 // Batched ensemble: many independent 2D runs (each its own context, grid and
 // parameters) advanced together.  Each phase of a stage has two plans:
 //   x-pass: one launch per distinct N1 (k_xpass_batch), or one launch for all
-//           runs (k_xpass_mixed, each work item dispatching on its run's N1);
+//           runs (k_xpass_mixed, each work item dispatching on its run's N1)
+//           with the default FFT plans or first radix 8 where N1 allows it;
 //   modal:  one launch per distinct rows-per-lane M (k_modal_batch), or one
 //           launch for all runs with 4 or 8 modes per block (k_modal_mixed).
 // Both kernels loop over the work items of all their runs (grid-stride, so a
@@ -949,7 +971,9 @@ struct Batch {
     Group mmix[3];
     int xplan = SF_BATCH_GROUPED;
     int mplan = SF_BATCH_GROUPED;
-    std::vector<double> tuned_ms;  // x grouped, x mixed, modal grouped, modal mixed 4, modal mixed 8
+    // x grouped, x mixed, x mixed with first radix 8, modal grouped, modal
+    // mixed with 4 and with 8 modes per block
+    std::vector<double> tuned_ms;
     std::vector<double> dts;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1040,13 +1064,15 @@ struct Batch {
         SF_HIP(hipMemcpy(g.start, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice));
     }
 
-    void build_xgroup(Group& g, int zch, int grid) {
+    // r1f: first radix of the FFT plans (mixed launches; 0 = default)
+    void build_xgroup(Group& g, int zch, int grid, int r1f = 0) {
         std::vector<XArgs<T>> a;
         std::vector<int> st;
         int total = 0;
         for (int r : g.members) {
             XArgs<T> x = runs[r]->xargs_2d(false, true);
             x.zch = zch;
+            x.tw = runs[r]->x_twiddles(r1f);
             a.push_back(x);
             st.push_back(total);
             total += (runs[r]->N3 + zch - 1) / zch;
@@ -1089,8 +1115,8 @@ struct Batch {
 
     void build() {
         release_graph();
-        if (xplan == SF_BATCH_MIXED) {
-            build_xgroup(xmix, xmix.zch, xmix.grid);
+        if (xplan != SF_BATCH_GROUPED) {
+            build_xgroup(xmix, xmix.zch, xmix.grid, xr1f());
         } else {
             for (auto& g : xg) build_xgroup(g, g.zch, g.grid);
         }
@@ -1104,15 +1130,17 @@ struct Batch {
         }
         dts.clear();
         for (auto* c : runs) dts.push_back(c->dt);
-        launches_per_step = 3 * ((xplan == SF_BATCH_MIXED ? 1 : int(xg.size())) +
+        launches_per_step = 3 * ((xplan != SF_BATCH_GROUPED ? 1 : int(xg.size())) +
                                  (mplan == SF_BATCH_GROUPED ? int(mg[0].size()) : 1));
     }
+
+    int xr1f() const { return xplan == SF_BATCH_MIXED_R8 ? 8 : 0; }
 
     void launch_x(bool mixed) {
         if (mixed) {
             const hipError_t e = launch_xpass_mixed<T>(static_cast<const XArgs<T>*>(xmix.args), xmix.start,
-                                                       int(xmix.members.size()), xmix.total, xmix.span, xmix.grid,
-                                                       stream);
+                                                       int(xmix.members.size()), xmix.total, xmix.span, xr1f(),
+                                                       xmix.grid, stream);
             if (e != hipSuccess) throw Error(SF_ERR_HIP, std::string("batched xpass: ") + hipGetErrorString(e));
             return;
         }
@@ -1139,7 +1167,7 @@ struct Batch {
     }
 
     void launch_stage(int k) {
-        launch_x(xplan == SF_BATCH_MIXED);
+        launch_x(xplan != SF_BATCH_GROUPED);
         launch_m(mplan, mg[k], mmix[k]);
     }
 
@@ -1157,21 +1185,21 @@ struct Batch {
     }
 
     void tune() {
-        tuned_ms.assign(5, 0.0);
+        tuned_ms.assign(6, 0.0);
         // x-pass: levels per block and resident blocks of each N1 group, and of
-        // the single launch over all runs
-        auto best_x = [&](Group& g, bool mixed) {
+        // the single launch over all runs under each FFT plan
+        auto best_x = [&](Group& g, bool mixed, int r1f) {
             double best = 1e30;
             int bz = 2, bgrid = 8 * ncu;
             for (int zch : {2, 4, 8, 16}) {
                 for (int mult : {4, 8, 16, 0}) {
                     const int grid = mult ? mult * ncu : (1 << 30);
-                    build_xgroup(g, zch, grid);
+                    build_xgroup(g, zch, grid, r1f);
                     const double ms = time_ms([&]() {
                         hipError_t e;
                         if (mixed)
                             e = launch_xpass_mixed<T>(static_cast<const XArgs<T>*>(g.args), g.start,
-                                                      int(g.members.size()), g.total, g.span, g.grid, stream);
+                                                      int(g.members.size()), g.total, g.span, r1f, g.grid, stream);
                         else
                             e = launch_xpass_batch<T>(static_cast<const XArgs<T>*>(g.args), g.start,
                                                       int(g.members.size()), g.total, g.key, g.grid, stream);
@@ -1188,9 +1216,18 @@ struct Batch {
             g.grid = bgrid;
             return best;
         };
-        for (auto& g : xg) tuned_ms[0] += best_x(g, false);
-        tuned_ms[1] = best_x(xmix, true);
-        xplan = (tuned_ms[1] < tuned_ms[0]) ? SF_BATCH_MIXED : SF_BATCH_GROUPED;
+        for (auto& g : xg) tuned_ms[0] += best_x(g, false, 0);
+        Group x8 = xmix;
+        tuned_ms[2] = best_x(x8, true, 8);
+        free_group(x8);
+        tuned_ms[1] = best_x(xmix, true, 0);
+        xplan = SF_BATCH_GROUPED;
+        if (tuned_ms[1] < tuned_ms[0]) xplan = SF_BATCH_MIXED;
+        if (tuned_ms[2] < std::min(tuned_ms[0], tuned_ms[1])) {
+            xplan = SF_BATCH_MIXED_R8;
+            xmix.zch = x8.zch;
+            xmix.grid = x8.grid;
+        }
 
         // modal: stage-0 launches on scratch copies of each run's state (the
         // fields the stage rewrites and its explicit-term outputs)
@@ -1226,7 +1263,7 @@ struct Batch {
                 g.start = nullptr;
                 build_mgroup(g, false, sargs);
             }
-            tuned_ms[2] = time_ms([&]() { launch_m(SF_BATCH_GROUPED, tg, mmix[0]); });
+            tuned_ms[3] = time_ms([&]() { launch_m(SF_BATCH_GROUPED, tg, mmix[0]); });
             for (auto& g : tg) free_group(g);
             Group tm = mmix[0];
             tm.args = nullptr;
@@ -1234,7 +1271,7 @@ struct Batch {
             for (int q = 0; q < 2; ++q) {
                 tm.key = q ? 8 : 4;
                 build_mgroup(tm, true, sargs);
-                tuned_ms[3 + q] = time_ms([&]() { launch_m(q ? SF_BATCH_MIXED8 : SF_BATCH_MIXED, tg, tm); });
+                tuned_ms[4 + q] = time_ms([&]() { launch_m(q ? SF_BATCH_MIXED8 : SF_BATCH_MIXED, tg, tm); });
             }
             free_group(tm);
             SF_HIP(hipStreamSynchronize(stream));
@@ -1244,17 +1281,18 @@ struct Batch {
         }
         release();
         mplan = SF_BATCH_GROUPED;
-        double best = tuned_ms[2];
-        if (tuned_ms[3] < best) {
-            best = tuned_ms[3];
+        double best = tuned_ms[3];
+        if (tuned_ms[4] < best) {
+            best = tuned_ms[4];
             mplan = SF_BATCH_MIXED;
         }
-        if (tuned_ms[4] < best) mplan = SF_BATCH_MIXED8;
+        if (tuned_ms[5] < best) mplan = SF_BATCH_MIXED8;
     }
 
     // a plan chosen by the caller instead of the tuner (-1 keeps a phase's)
     void select(int x, int m) {
-        SF_REQUIRE(x == -1 || x == SF_BATCH_GROUPED || x == SF_BATCH_MIXED, SF_ERR_ARG, "bad x-pass plan");
+        SF_REQUIRE(x == -1 || x == SF_BATCH_GROUPED || x == SF_BATCH_MIXED || x == SF_BATCH_MIXED_R8, SF_ERR_ARG,
+                   "bad x-pass plan");
         SF_REQUIRE(m == -1 || m == SF_BATCH_GROUPED || m == SF_BATCH_MIXED || m == SF_BATCH_MIXED8, SF_ERR_ARG,
                    "bad modal plan");
         if (x != -1) xplan = x;
@@ -2069,7 +2107,7 @@ int sf_batch_plan(const sf_batch* b, int* xpass_plan, int* modal_plan, double* t
         if (xpass_plan) *xpass_plan = B.xplan;
         if (modal_plan) *modal_plan = B.mplan;
         if (tuned_ms)
-            for (int i = 0; i < 5; ++i) tuned_ms[i] = B.tuned_ms[i];
+            for (int i = 0; i < 6; ++i) tuned_ms[i] = B.tuned_ms[i];
     });
 }
 

@@ -194,22 +194,31 @@ __device__ __forceinline__ void reg_dft(cx<T>* v) {
 }
 
 // ---- plan ------------------------------------------------------------------
-template <int N>
+// R1F = 0: the default first radix (16 where N/16 <= 32, else 8); R1F = 8
+// or 16 asks for that first radix where it gives R2 <= 32 (fft_plan_ok), the
+// default otherwise.
+template <int N, int R1F>
+constexpr bool fft_plan_ok() {
+    return N > 32 && (R1F == 8 || R1F == 16) && N % R1F == 0 && N / R1F <= 32;
+}
+template <int N, int R1F = 0>
 struct FftPlan {
     static constexpr bool two = N > 32;
-    static constexpr int R1 = !two ? N : ((N % 16 == 0 && N / 16 <= 32) ? 16 : 8);
+    static constexpr int R1 = !two ? N
+                              : fft_plan_ok<N, R1F>() ? R1F
+                                                      : ((N % 16 == 0 && N / 16 <= 32) ? 16 : 8);
     static constexpr int R2 = N / R1;
     static_assert(!two || (R1 * R2 == N && R2 <= 32), "unsupported FFT length");
 };
 
 // LDS line stride: room for the padded pass-1 table, staggered by half a bank
 // cycle (16 complex values) between consecutive lines
-template <int N>
+template <int N, int R1F = 0>
 constexpr int fft_ldp() {
     if constexpr (N <= 32) {
         return N + 1;
     } else {
-        int l = N + FftPlan<N>::R2;
+        int l = N + FftPlan<N, R1F>::R2;
         while (l % 32 != 16) ++l;
         return l;
     }
@@ -219,10 +228,10 @@ constexpr int fft_ldp() {
 // ES = line count when element i of line l sits at lp(l) + i*ES).  IL: work
 // items run over the lines first (consecutive lanes take consecutive lines,
 // which is what keeps interleaved lines free of LDS bank conflicts).
-template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, typename LP>
+template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, int R1F = 0, typename LP>
 __device__ __forceinline__ void fft_pass1(int nl, LP lp, const int tid) {
     using V = cx<T>;
-    constexpr int R1 = FftPlan<N>::R1, R2 = FftPlan<N>::R2;
+    constexpr int R1 = FftPlan<N, R1F>::R1, R2 = FftPlan<N, R1F>::R2;
     constexpr int PER = (NLMAX * R2 + BS - 1) / BS;
     const int tot = nl * R2;
     V v[PER][R1];
@@ -255,10 +264,10 @@ __device__ __forceinline__ void fft_pass1(int nl, LP lp, const int tid) {
 
 // pass 2 twiddles: tw[j*R2 + r] = exp(-2 pi i r j / N), j < R1, r < R2 (each
 // lane reads its R2 - 1 twiddles from one contiguous run, in wide loads)
-template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, typename LP>
+template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, int R1F = 0, typename LP>
 __device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict__ tw, const int tid) {
     using V = cx<T>;
-    constexpr int R1 = FftPlan<N>::R1, R2 = FftPlan<N>::R2;
+    constexpr int R1 = FftPlan<N, R1F>::R1, R2 = FftPlan<N, R1F>::R2;
     constexpr int PER = (NLMAX * R1 + BS - 1) / BS;
     const int tot = nl * R1;
     V v[PER][R2];
@@ -330,12 +339,12 @@ __device__ __forceinline__ void fft_single(int nl, LP lp, const int tid) {
 // filling the lines; on return the results are visible to all threads.  tid
 // is the thread's index in [0, BS) for the distribution of the work items (a
 // block may permute it to spread partially filled rounds over its waves).
-template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, typename LP>
+template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, int R1F = 0, typename LP>
 __device__ __forceinline__ void lds_fft_lines(int nl, LP lp, const cx<T>* __restrict__ tw,
                                               const int tid = int(threadIdx.x)) {
     if constexpr (FftPlan<N>::two) {
-        fft_pass1<T, N, NLMAX, BS, SIGN, ES, IL>(nl, lp, tid);
-        fft_pass2<T, N, NLMAX, BS, SIGN, ES, IL>(nl, lp, tw, tid);
+        fft_pass1<T, N, NLMAX, BS, SIGN, ES, IL, R1F>(nl, lp, tid);
+        fft_pass2<T, N, NLMAX, BS, SIGN, ES, IL, R1F>(nl, lp, tw, tid);
     } else {
         fft_single<T, N, NLMAX, BS, SIGN, ES>(nl, lp, tid);
         (void)tw;
@@ -349,15 +358,16 @@ __device__ __forceinline__ void lds_fft(cx<T>* buf, const cx<T>* __restrict__ tw
     lds_fft_lines<T, N, NL, BS, SIGN>(NL, [buf](int line) { return buf + line * LDP; }, tw, tid);
 }
 
-// host: twiddle table of length N in the layout fft_pass2 reads (entry count N)
-template <int N>
+// host: twiddle table of length N in the layout fft_pass2 reads (entry count
+// N) for the plan FftPlan<N, R1F>
+template <int N, int R1F = 0>
 void fft_twiddle_table(double* re, double* im) {
-    constexpr int R1 = FftPlan<N>::R1, R2 = FftPlan<N>::R2;
+    constexpr int R1 = FftPlan<N, R1F>::R1, R2 = FftPlan<N, R1F>::R2;
     for (int k = 0; k < N; ++k) {
         re[k] = 1.0;
         im[k] = 0.0;
     }
-    if constexpr (FftPlan<N>::two) {
+    if constexpr (FftPlan<N, R1F>::two) {
         for (int j = 0; j < R1; ++j)
             for (int r = 0; r < R2; ++r) {
                 const double a = -2.0 * cpi * double((r * j) % N) / double(N);

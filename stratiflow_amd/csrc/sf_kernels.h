@@ -89,9 +89,11 @@ hipError_t launch_xpass_batch(const XArgs<T>* args, const int* start, int nruns,
 template <typename T>
 hipError_t launch_modal_mixed(const ModalArgs<T>* args, const int* start, int nruns, int total, int mb, int mmax,
                               int grid, hipStream_t s);
+// (r1f: first radix of the FFT plans, 0 for the defaults or 8; each run's tw
+// must be the table of its N1 under that plan, fft_twiddles_plan)
 template <typename T>
-hipError_t launch_xpass_mixed(const XArgs<T>* args, const int* start, int nruns, int total, int nmax, int grid,
-                              hipStream_t s);
+hipError_t launch_xpass_mixed(const XArgs<T>* args, const int* start, int nruns, int total, int nmax, int r1f,
+                              int grid, hipStream_t s);
 template <typename T>
 hipError_t launch_yinv(const YArgs<T>& a, int N2, hipStream_t s);
 template <typename T>
@@ -173,10 +175,15 @@ template <typename T>
 hipError_t launch_add_mean(cx<T>* f, int N3, int A1, T value, hipStream_t s);
 
 bool tz_supported(int n);
-// z levels per x-pass block for this grid (see sf_kernels.hip)
+// z levels per x-pass block for this grid (the block height that minimises
+// rounds of resident blocks x (levels + 2))
 int xpass_levels(int N1, int N2, int N3, bool threeD, int precision);
 // twiddle table of an n-point LDS FFT (the layout its second pass reads)
 void fft_twiddles(int n, double* re, double* im);
+// the same for the plan with first radix r1f (8 or 16 where n allows it,
+// else the default), and that plan's first radix
+void fft_twiddles_plan(int n, int r1f, double* re, double* im);
+int fft_first_radix(int n, int r1f);
 int max_rows_per_lane();
 // blocks (and so partial sums) of the two-pass FP64 reductions (dot, max-abs)
 int max_reduction_blocks();

@@ -614,9 +614,9 @@ bool tz_supported(int n) {
 
 // line pitch of the x-pass: the smallest that holds the FFT's padded pass-1
 // table, rounded to 8 mod 32 complex values (keeps four 3D blocks per CU)
-template <int N>
+template <int N, int R1F = 0>
 constexpr int x_ldp() {
-    int l = FftPlan<N>::two ? N + FftPlan<N>::R2 : N + 1;
+    int l = FftPlan<N, R1F>::two ? N + FftPlan<N, R1F>::R2 : N + 1;
     while (l % 32 != 8) ++l;
     return l;
 }
@@ -636,15 +636,17 @@ constexpr int xpass_bs() {
 
 // LDS values of one x-pass block: NP product lines and a ring of 3 x NI
 // input lines at the line pitch x_ldp
-template <int N1, bool THREED, bool LIN>
+template <int N1, bool THREED, bool LIN, int R1F = 0>
 constexpr int xpass_lds() {
-    return ((THREED ? 6 : 3) + 3 * (LIN ? 4 : 2)) * x_ldp<N1>();
+    return ((THREED ? 6 : 3) + 3 * (LIN ? 4 : 2)) * x_ldp<N1, R1F>();
 }
 
-template <typename T, int N1, bool THREED, bool LIN>
+// R1F: the FFT plan's first radix (0: the default; A.tw must be the table of
+// the same plan)
+template <typename T, int N1, bool THREED, bool LIN, int R1F = 0>
 __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, const int n2, cx<T>* const lds) {
     using V = cx<T>;
-    constexpr int LDP = x_ldp<N1>();
+    constexpr int LDP = x_ldp<N1, R1F>();
     constexpr int NI = LIN ? 4 : 2;        // input pair lines per level
     constexpr int NP = THREED ? 6 : 3;     // product pair lines per level
     constexpr int NB = NP + NI;            // lines per batched FFT
@@ -725,7 +727,8 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
     // the previous level's store, consumed by this level's store
     if (zbeg + 1 < zend) load(zbeg + 2);
     sync_lds();
-    lds_fft_lines<T, N1, 3 * NI, XB, -1>(3 * NI, [ring](int line) { return ring + line * LDP; }, A.tw, tid);
+    lds_fft_lines<T, N1, 3 * NI, XB, -1, 1, false, R1F>(3 * NI, [ring](int line) { return ring + line * LDP; }, A.tw,
+                                                        tid);
 
     for (int z = zbeg; z < zend; ++z) {
         const bool more = z + 1 < zend;
@@ -830,7 +833,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
         sync_lds();
         {
             const int s2 = slot(z + 2) * NI;
-            lds_fft_lines<T, N1, NB, XB, -1>(more ? NB : NP,
+            lds_fft_lines<T, N1, NB, XB, -1, 1, false, R1F>(more ? NB : NP,
                                              [prd, s2](int line) {
                                                  return prd + (line + (line >= NP ? s2 : 0)) * LDP;
                                              },
@@ -891,27 +894,29 @@ __global__ __launch_bounds__(BS) void k_xpass_batch(const XArgs<T>* __restrict__
 
 // 2D runs of every N1 up to NMAX in one launch: each work item dispatches on
 // its run's N1 and all cases share one LDS allocation sized for the largest.
-template <int NMAX>
+// R1F picks the FFT plan of every N1 that has it (fft_plan_ok), the default
+// plan of the others; each run's tw is the table of its plan.
+template <int NMAX, int R1F>
 constexpr int xpass_mixed_lds() {
     int m = 0;
 #define SF_CASE(n) \
-    if (n <= NMAX && xpass_lds<n, false, false>() > m) m = xpass_lds<n, false, false>();
+    if (n <= NMAX && xpass_lds<n, false, false, R1F>() > m) m = xpass_lds<n, false, false, R1F>();
     SF_SIZES(SF_CASE)
 #undef SF_CASE
     return m;
 }
 
-template <typename T, int NMAX>
+template <typename T, int NMAX, int R1F>
 __global__ __launch_bounds__(BS) void k_xpass_mixed(const XArgs<T>* __restrict__ args, const int* __restrict__ start,
                                                     int nruns, int total) {
-    __shared__ cx<T> lds[xpass_mixed_lds<NMAX>()];
+    __shared__ cx<T> lds[xpass_mixed_lds<NMAX, R1F>()];
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const int r = find_run(start, nruns, item);
         const XArgs<T>& a = args[r];
         switch (a.N1) {
-#define SF_CASE(n)                                                                    \
-    case n:                                                                           \
-        if constexpr (n <= NMAX) xpass_body<T, n, false, false>(a, item - start[r], 0, lds); \
+#define SF_CASE(n)                                                                              \
+    case n:                                                                                     \
+        if constexpr (n <= NMAX) xpass_body<T, n, false, false, R1F>(a, item - start[r], 0, lds); \
         break;
             SF_SIZES(SF_CASE)
 #undef SF_CASE
@@ -921,14 +926,23 @@ __global__ __launch_bounds__(BS) void k_xpass_mixed(const XArgs<T>* __restrict__
     }
 }
 
-// the variant compiled for N1 up to 128, 256 or 512 that covers nmax
+template <typename T, int R1F>
+static void launch_xpass_mixed_r(const XArgs<T>* args, const int* start, int nruns, int total, int nmax, int grid,
+                                 hipStream_t s) {
+    if (nmax <= 128) hipLaunchKernelGGL((k_xpass_mixed<T, 128, R1F>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total);
+    else if (nmax <= 256) hipLaunchKernelGGL((k_xpass_mixed<T, 256, R1F>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total);
+    else hipLaunchKernelGGL((k_xpass_mixed<T, 512, R1F>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total);
+}
+
+// the variant compiled for N1 up to 128, 256 or 512 that covers nmax, with
+// the default FFT plans (r1f = 0) or first radix r1f = 8 where possible
 template <typename T>
-hipError_t launch_xpass_mixed(const XArgs<T>* args, const int* start, int nruns, int total, int nmax, int grid,
-                              hipStream_t s) {
+hipError_t launch_xpass_mixed(const XArgs<T>* args, const int* start, int nruns, int total, int nmax, int r1f,
+                              int grid, hipStream_t s) {
     if (total <= 0) return hipSuccess;
-    if (nmax <= 128) hipLaunchKernelGGL((k_xpass_mixed<T, 128>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total);
-    else if (nmax <= 256) hipLaunchKernelGGL((k_xpass_mixed<T, 256>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total);
-    else if (nmax <= 512) hipLaunchKernelGGL((k_xpass_mixed<T, 512>), dim3(grid), dim3(BS), 0, s, args, start, nruns, total);
+    if (nmax > 512) return hipErrorInvalidValue;
+    if (r1f == 0) launch_xpass_mixed_r<T, 0>(args, start, nruns, total, nmax, grid, s);
+    else if (r1f == 8) launch_xpass_mixed_r<T, 8>(args, start, nruns, total, nmax, grid, s);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
@@ -1036,6 +1050,33 @@ int xpass_levels(int N1, int N2, int N3, bool threeD, int precision) {
 #endif
 
 #ifdef SF_KERNELS_HOST_DEFS
+void fft_twiddles_plan(int n, int r1f, double* re, double* im) {
+    switch (n) {
+#define SF_CASE(n)                              \
+    case n:                                     \
+        if (r1f == 8) fft_twiddle_table<n, 8>(re, im);   \
+        else if (r1f == 16) fft_twiddle_table<n, 16>(re, im); \
+        else fft_twiddle_table<n>(re, im);      \
+        return;
+        SF_SIZES(SF_CASE)
+#undef SF_CASE
+        default:
+            return;
+    }
+}
+
+int fft_first_radix(int n, int r1f) {
+    switch (n) {
+#define SF_CASE(n)                                                                       \
+    case n:                                                                              \
+        return r1f == 8 ? FftPlan<n, 8>::R1 : (r1f == 16 ? FftPlan<n, 16>::R1 : FftPlan<n>::R1);
+        SF_SIZES(SF_CASE)
+#undef SF_CASE
+        default:
+            return 0;
+    }
+}
+
 void fft_twiddles(int n, double* re, double* im) {
     switch (n) {
 #define SF_CASE(n)                           \
@@ -1880,7 +1921,7 @@ hipError_t launch_diffprec(const DiffPrecArgs<T>& p, hipStream_t s) {
 #define SF_INST_BATCH(T) \
     template hipError_t launch_modal_batch<T>(const ModalArgs<T>*, const int*, int, int, int, int, hipStream_t); \
     template hipError_t launch_xpass_batch<T>(const XArgs<T>*, const int*, int, int, int, int, hipStream_t); \
-    template hipError_t launch_xpass_mixed<T>(const XArgs<T>*, const int*, int, int, int, int, hipStream_t); \
+    template hipError_t launch_xpass_mixed<T>(const XArgs<T>*, const int*, int, int, int, int, int, hipStream_t); \
     template hipError_t launch_modal_mixed<T>(const ModalArgs<T>*, const int*, int, int, int, int, int, hipStream_t);
 
 }  // namespace sf

@@ -2,7 +2,11 @@
 
 Each batched run must end bit-identical to a twin context stepped on its own
 (TimeStep, IMEXRK.cpp:8-24): the batched kernels run the same per-level and
-per-mode code, only the assignment of work to blocks differs.
+per-mode code, only the assignment of work to blocks differs.  The exception
+is an x-pass FFT plan other than the runs' own (first radix 8): it rounds
+differently, as FFTW's timed plans do (FFTW_PATIENT, Field.h:730-991), so those
+runs agree with their twins to rounding (FP32 1e-5, FP64 1e-12 of the field's
+largest value after a few steps, ten times that for the pressure).
 """
 import numpy as np
 import pytest
@@ -12,6 +16,15 @@ pytestmark = pytest.mark.gpu
 # (N1, N3, Re, Ri, dt): uneven grids and parameters, two x-pass groups share N1
 RUNS = [(32, 48, 500.0, 0.05, 0.01), (48, 64, 1200.0, 0.12, 0.01), (32, 96, 2000.0, 0.2, 0.005),
         (64, 128, 4000.0, 0.25, 0.01), (48, 160, 800.0, 0.1, 0.008), (96, 64, 3000.0, 0.15, 0.01)]
+
+
+def same(a, c, exact, prec, field="u1"):
+    """Bit-identical, or within rounding of the other FFT plan (ten times
+    looser for the pressure, which the projection rescales by 1/h)."""
+    if exact:
+        return np.array_equal(a, c)
+    tol = (1e-5 if prec == 4 else 1e-12) * (10 if field == "p" else 1)
+    return np.abs(a - c).max() <= tol * np.abs(c).max()
 
 
 def _make(N1, N3, Re, Ri, dt, seed, prec):
@@ -39,6 +52,7 @@ def test_batch_matches_individual_runs(prec):
     info = b.info()
     print("batch", info, flush=True)
     assert info["xpass_groups"] == 4 and info["modal_groups"] == 3
+    exact = info["xpass_plan"] != "mixed_r8"
     b.step(5)
     for t in twins:
         t.TimeStep(5)
@@ -49,7 +63,7 @@ def test_batch_matches_individual_runs(prec):
         for f in ("u1", "u3", "b", "p"):
             a, c = s.get(f), t.get(f)
             assert np.isfinite(a).all()
-            assert np.array_equal(a, c), (i, f, np.abs(a - c).max() / np.abs(c).max())
+            assert same(a, c, exact, prec, f), (i, f, np.abs(a - c).max() / np.abs(c).max())
     # a dt change is picked up (the graph is rebuilt)
     batched[0].deltaT = 0.004
     twins[0].deltaT = 0.004
@@ -57,14 +71,15 @@ def test_batch_matches_individual_runs(prec):
     for t in twins:
         t.TimeStep(2)
     for s, t in zip(batched, twins):
-        assert np.array_equal(s.get("u1"), t.get("u1"))
+        assert same(s.get("u1"), t.get("u1"), exact, prec)
     b.close()
 
 
 @pytest.mark.parametrize("prec", [4, 8])
 def test_every_batch_plan_matches_individual_runs(prec):
-    """One launch per N1 / per vertical tile, or one launch for all runs (with
-    4 or 8 modes per modal block): the tuner's choice changes the time only."""
+    """One launch per N1 / per vertical tile, or one launch for all runs (the
+    x-pass with the default FFT plans or first radix 8, the modal kernel with 4
+    or 8 modes per block): the tuner's choice changes the time only."""
     import stratiflow_amd as sf
     batched = [_make(*r, seed=200 + i, prec=prec) for i, r in enumerate(RUNS)]
     twins = [_make(*r, seed=200 + i, prec=prec) for i, r in enumerate(RUNS)]
@@ -73,16 +88,18 @@ def test_every_batch_plan_matches_individual_runs(prec):
     print("tuned", info, flush=True)
     t = info["tuned_ms_per_stage"]
     assert all(v > 0 for v in t.values())
-    xbest = "mixed" if t["xpass_mixed"] < t["xpass_grouped"] else "grouped"
-    mbest = min(("grouped", "mixed", "mixed8"), key=lambda k: t[{"grouped": "modal_grouped", "mixed": "modal_mixed4",
-                                                                "mixed8": "modal_mixed8"}[k]])
+    xbest = min(("grouped", "mixed", "mixed_r8"), key=lambda k: t["xpass_" + k])
+    mbest = min(("grouped", "mixed4", "mixed8"), key=lambda k: t["modal_" + k])
     assert (info["xpass_plan"], info["modal_plan"]) == (xbest, mbest)
-    plans = [(sf.BATCH_MIXED, sf.BATCH_MIXED), (sf.BATCH_GROUPED, sf.BATCH_MIXED8), (sf.BATCH_MIXED, sf.BATCH_GROUPED),
-             (sf.BATCH_GROUPED, sf.BATCH_GROUPED), (sf.BATCH_MIXED, sf.BATCH_MIXED8)]
+    # the plans with the runs' own FFT plans first (bit for bit), then first radix 8
+    plans = [(sf.BATCH_MIXED, sf.BATCH_MIXED), (sf.BATCH_GROUPED, sf.BATCH_MIXED8), (sf.BATCH_GROUPED, sf.BATCH_GROUPED),
+             (sf.BATCH_MIXED, sf.BATCH_MIXED8), (sf.BATCH_MIXED_R8, sf.BATCH_GROUPED), (sf.BATCH_MIXED_R8, sf.BATCH_MIXED)]
+    exact = True
     for x, m in plans:
         b.select(x, m)
         info = b.info()
         assert info["launches_per_step"] == 3 * ((1 if x else 4) + (1 if m else 3)), info
+        exact = exact and x != sf.BATCH_MIXED_R8
         b.step(2)
         for tw in twins:
             tw.TimeStep(2)
@@ -90,5 +107,5 @@ def test_every_batch_plan_matches_individual_runs(prec):
             for f in ("u1", "u3", "b", "p"):
                 a, c = s.get(f), tw.get(f)
                 assert np.isfinite(a).all()
-                assert np.array_equal(a, c), (x, m, i, f)
+                assert same(a, c, exact, prec, f), (x, m, i, f, np.abs(a - c).max() / np.abs(c).max())
     b.close()
