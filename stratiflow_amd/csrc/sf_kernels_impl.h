@@ -136,12 +136,16 @@ constexpr int modal_mb() {
 // memory sees runs of MB consecutive j1 ([ja][z][j1] layout); wave w then
 // holds the column of mode j10+w with lane l owning rows l*M .. l*M+M-1.
 // Waves past A1 run on zeros and store nothing; they keep the block's barriers.
-template <typename T, int M, int MB>
+// FULL (N3 == 64*M): every tile row exists, so a thread keeps one base offset
+// per table and reaches its row t through the scalar offset (fewer live
+// registers than an offset per row).
+template <typename T, int M, int MB, bool FULL = false>
 __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, const int j10, cx<T>* const tile) {
     using V = cx<T>;
     constexpr int TP = MB + 1;  // tile row pitch (tile: 64*M*TP values of LDS)
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // wave-uniform: the mode's scalars (j1, wavenumbers, flags) live in SGPRs
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     const int A1 = A.A1;
     const bool active = j10 + wave < A1;
     const int j1 = active ? j10 + wave : A1 - 1;
@@ -158,10 +162,17 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
     // coefficient table: row c of the lane's row t through the scalar offset
     // c * N3p; rows at or beyond N3 read zero
     const rsrc_t zr = make_rsrc(A.zc, size_t(ZC_COUNT) * A.N3p * sizeof(T));
-    int zoff[M];
+    int zoff[FULL ? 1 : M];
+    if constexpr (FULL) {
+        zoff[0] = lane * M * int(sizeof(T));
+    } else {
 #pragma unroll
-    for (int t = 0; t < M; ++t) zoff[t] = row[t] < N ? row[t] * int(sizeof(T)) : BUF_OOB;
-    auto coefz = [&](int c, int t) -> T { return bld<T>(zr, zoff[t], c * A.N3p * int(sizeof(T))); };
+        for (int t = 0; t < M; ++t) zoff[t] = row[t] < N ? row[t] * int(sizeof(T)) : BUF_OOB;
+    }
+    auto coefz = [&](int c, int t) -> T {
+        if constexpr (FULL) return bld<T>(zr, zoff[0], (c * A.N3p + t) * int(sizeof(T)));
+        else return bld<T>(zr, zoff[t], c * A.N3p * int(sizeof(T)));
+    };
     // tile element owned by this thread in the cooperative (coalesced) phase:
     // rows z = tid/MB + t*64 for t < M (N3 <= 64*M), mode w = tid % MB.  Every
     // modal field has the same layout, so one table of byte offsets serves all
@@ -171,12 +182,23 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
     const int cz = int(threadIdx.x) / MB;
     const int cj1 = j10 + cw;
     const size_t fbytes = size_t(A1) * size_t(A.A2) * size_t(N) * sizeof(V);
-    int foff[M];
+    int foff[FULL ? 1 : M];
+    if constexpr (FULL) {
+        foff[0] = (cj1 < A1) ? int(sidx(ja, cz, cj1, N, A1) * sizeof(V)) : BUF_OOB;
+    } else {
 #pragma unroll
-    for (int t = 0; t < M; ++t) {
-        const int z = cz + t * 64;
-        foff[t] = (z < N && cj1 < A1) ? int(sidx(ja, z, cj1, N, A1) * sizeof(V)) : BUF_OOB;
+        for (int t = 0; t < M; ++t) {
+            const int z = cz + t * 64;
+            foff[t] = (z < N && cj1 < A1) ? int(sidx(ja, z, cj1, N, A1) * sizeof(V)) : BUF_OOB;
+        }
     }
+    // element t: vector offset fo(t) plus scalar offset fs(t) (64 rows of A1 modes per t)
+    const int fstride = 64 * A1 * int(sizeof(V));
+    auto fo = [&](int t) -> int {
+        if constexpr (FULL) return foff[0];
+        else return foff[t];
+    };
+    auto fs = [&](int t) -> int { return FULL ? t * fstride : 0; };
     // issue: loads of this thread's tile elements into g (left in flight);
     // finish: through the LDS tile into the wave's column x.  Several issues
     // before their finishes overlap the memory latency of those fields.
@@ -184,7 +206,7 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         if (f == nullptr) return;  // block-uniform
         const rsrc_t r = make_rsrc(f, fbytes);
 #pragma unroll
-        for (int t = 0; t < M; ++t) g[t] = bld<V>(r, foff[t]);
+        for (int t = 0; t < M; ++t) g[t] = bld<V>(r, fo(t), fs(t));
     };
     auto finish = [&](const V* f, const V (&g)[M], V (&x)[M]) {
         if (f == nullptr) {  // block-uniform
@@ -209,7 +231,7 @@ __device__ __forceinline__ void modal_body(const ModalArgs<T>& A, const int ja, 
         for (int t = 0; t < M; ++t) tile[row[t] * TP + wave] = x[t];
         sync_lds();
 #pragma unroll
-        for (int t = 0; t < M; ++t) bst<V>(tile[(cz + t * 64) * TP + cw], r, foff[t]);
+        for (int t = 0; t < M; ++t) bst<V>(tile[(cz + t * 64) * TP + cw], r, fo(t), fs(t));
     };
     // ZeroEnds, Field.h:475-486
     auto zero_ends = [&](V (&x)[M], bool dir) {
@@ -443,13 +465,13 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (b % NXCD) * (nb / NXCD) + b / NXCD;
 }
 
-template <typename T, int M>
+template <typename T, int M, bool FULL>
 __global__ __launch_bounds__((modal_mb<T, M>() * 64)) void k_modal(ModalArgs<T> A) {
     constexpr int MB = modal_mb<T, M>();
     __shared__ cx<T> tile[64 * M * (MB + 1)];
     const int nt = gridDim.x;  // j1 tiles per ja
     const int t = xcd_tile(blockIdx.x + nt * blockIdx.y, nt * gridDim.y);
-    modal_body<T, M, MB>(A, t / nt, (t % nt) * MB, tile);
+    modal_body<T, M, MB, FULL>(A, t / nt, (t % nt) * MB, tile);
 }
 
 // index r of the run owning work item `item`: start[r] <= item < start[r+1]
@@ -567,11 +589,13 @@ template <typename T>
 hipError_t launch_modal(const ModalArgs<T>& a, hipStream_t s) {
     const int M = (a.N3 + 63) / 64;
     switch (M) {
-#define SF_M(m)                                                                                       \
-    case m: {                                                                                         \
-        constexpr int MB = modal_mb<T, m>();                                                          \
-        hipLaunchKernelGGL((k_modal<T, m>), dim3((a.A1 + MB - 1) / MB, a.A2), dim3(MB * 64), 0, s, a); \
-        break;                                                                                        \
+#define SF_M(m)                                                                                              \
+    case m: {                                                                                                \
+        constexpr int MB = modal_mb<T, m>();                                                                 \
+        const dim3 grid((a.A1 + MB - 1) / MB, a.A2);                                                         \
+        if (a.N3 == 64 * m) hipLaunchKernelGGL((k_modal<T, m, true>), grid, dim3(MB * 64), 0, s, a);         \
+        else hipLaunchKernelGGL((k_modal<T, m, false>), grid, dim3(MB * 64), 0, s, a);                       \
+        break;                                                                                               \
     }
         SF_M(1) SF_M(2) SF_M(3) SF_M(4) SF_M(5) SF_M(6) SF_M(7) SF_M(8)
 #undef SF_M
