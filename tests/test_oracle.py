@@ -534,3 +534,40 @@ def test_evolve_background_matches_dense_crank_nicolson():
     assert np.allclose(U1, want, rtol=0, atol=1e-13)
     # and it diffuses: the interior profile moves toward its mean
     assert np.abs(np.diff(U1[1:N3 - 1])).max() < np.abs(np.diff(U[1:N3 - 1])).max()
+
+
+# ------------------------------------------------------- LinearEvolve dt
+@pytest.mark.parametrize("T", [5.0, 37.0])
+def test_linear_evolve_time_steps_closed_form(T):
+    """LinearEvolve's bookkeeping (StateVector.cpp:133-146) about the zero state:
+    CFLlinear is dt*N1/L1 there (IMEXRK.h:376-393), so the first rescale sets
+    dt = 0.4*L1/N1 and, every 100 steps, keeps it; once fewer than 100 steps are
+    left, dt becomes remaining/(int(remaining/dt) + 1).  The step sequence of
+    the restatement must be exactly that."""
+    from oracle import krylov
+    N1, N3 = 24, 32
+    o = make(N1=N1, N3=N3)
+    L1 = 2 * np.pi / 0.44
+    dts = []
+    step = o.step_linear
+
+    def record(n):
+        dts.append(o.dt)
+        step(n)
+
+    o.step_linear = record
+    zero = {n: np.zeros(o.modal_shape, dtype=complex) for n in ("u1", "u3", "b")}
+    krylov.linear_evolve(o, dict(zero), dict(zero), T)
+    d = 0.4 * L1 / N1
+    expect, t = [], 0.0
+    while t + 1e-4 < T:
+        if len(expect) % 100 == 0:
+            dd = d
+            left = int((T - t) / dd) + 1
+            if left < 100:
+                dd = (T - t) / left
+        expect.append(dd)
+        t += dd
+    assert len(dts) == len(expect)
+    assert np.allclose(dts, expect, rtol=1e-12, atol=0)
+    assert abs(sum(dts) - T) < 1e-9
