@@ -17,8 +17,9 @@
 //   learned  the network on the analytic base, judged by plain GMRES (a
 //            preconditioned solve that misses the tolerance is redone plain).
 // usage: newton_bench key=value ...  (N1 N3 T dt K tol newton ntrain ntest
-//        batches delta amp seed methods; methods is a mask: 1 plain, 2 learned,
-//        4 base)
+//        batches delta amp seed methods first; methods is a mask: 1 plain,
+//        2 learned, 4 base; the held-out problems are first .. first+ntest-1,
+//        so a large set can be measured in parts)
 // Prints one line each:
 //   train <i> <Re> <Ri> <newton steps> <matvecs> <seconds> <samples so far> <|F|/|F0|>
 //   epoch <e> <loss> <seconds>
@@ -109,7 +110,8 @@ int main(int argc, char** argv) {
     setvbuf(stdout, nullptr, _IOLBF, 0);
     std::map<std::string, double> o = {{"N1", 256}, {"N3", 256}, {"T", 2}, {"dt", 0.01}, {"K", 61}, {"tol", 1e-2},
                                        {"newton", 12}, {"ntrain", 3}, {"ntest", 3}, {"batches", 4000},
-                                       {"delta", 0.3}, {"amp", 1e-6}, {"seed", 7000}, {"methods", 7}};
+                                       {"delta", 0.3}, {"amp", 1e-6}, {"seed", 7000}, {"methods", 7},
+                                       {"first", 0}};
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         const size_t e = a.find('=');
@@ -199,7 +201,7 @@ int main(int argc, char** argv) {
     }
 
     // held-out problems, three methods from the same start
-    for (int i = 0; i < int(o["ntest"]); ++i) {
+    for (int i = int(o["first"]); i < int(o["first"]) + int(o["ntest"]); ++i) {
         Problem p = make_problem(seed + 1000 + i, o["amp"]);
         use(p);
         const double r0 = residual(p.x0);
