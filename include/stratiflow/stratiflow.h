@@ -560,6 +560,9 @@ inline void VerticalPointsBoth(std::vector<double>& z, std::vector<double>& zf) 
     sfacade::check(sf_vertical_points(sfacade::ctx(), z.data(), zf.data()), sfacade::ctx());
 }
 
+// HeatPlot, as the reference's IMEXRK.h includes its Graph.h
+#include "Graph.h"
+
 // Field.h:1072-1116: coefficients (j1^2+j2^2+h^2+1)^(-5/3) (U(-1,1) + i U(-1,1))
 // on the profiles exp(-2 z^2) H_h(2 z), h < 4, for j1 < N1/6 and |j2| <= M2
 // (M2 = N2/6 in 3D, N2/2 otherwise), drawn from the facade's generator.
@@ -835,13 +838,17 @@ public:
         UpdateForTimestep();
         return stratifloat(cfl);
     }
-    // IMEXRK.h:164-215: PrepareRun, PrepareRunLinear and PrepareRunAdjoint
-    // populate the nodal fields; the image folders of the reference's plots are
-    // not made (nothing is drawn)
+    // IMEXRK.h:164-218: PrepareRun, PrepareRunLinear and PrepareRunAdjoint
+    // populate the nodal fields and, when makeDirs, make the image folders the
+    // Plot* members write into
     void PrepareRun(const std::string& imageDir = "", bool makeDirs = true) {
-        (void)imageDir;
-        (void)makeDirs;
+        imageDirectory = imageDir;
         PopulateNodalVariables();
+        if (makeDirs) {
+            for (const char* d : {"/u1", "/u2", "/u3", "/buoyancy", "/pressure", "/vorticity", "/perturbvorticity",
+                                  "/buoyancyBG"})
+                MakeCleanDir(imageDirectory + d);
+        }
     }
     void PrepareRunLinear(const std::string& imageDir = "", bool makeDirs = true) { PrepareRun(imageDir, makeDirs); }
     stratifloat KE() const {
@@ -881,6 +888,69 @@ public:
         B.ToModal(b);
     }
 
+    // IMEXRK.h:221-303: heat maps of slice j2 (Graph.h); the nodal fields are
+    // refreshed from the modal state first
+    void PlotBuoyancy(std::string filename, int j2, bool includeBackground = true) {
+        if (includeBackground) {
+            Neumann1D B_;
+            B_.SetValue([](stratifloat z) { return z; }, flowParams.L3);
+            NeumannNodal nn;
+            nodal(b, nn);
+            nn += B_;
+            NeumannModal nm;
+            nn.ToModal(nm);
+            HeatPlot(nm, flowParams.L1, flowParams.L3, j2, filename);
+        } else {
+            HeatPlot(b, flowParams.L1, flowParams.L3, j2, filename);
+        }
+    }
+    void PlotPressure(std::string filename, int j2) { HeatPlot(p, flowParams.L1, flowParams.L3, j2, filename); }
+    void PlotVerticalVelocity(std::string filename, int j2) { HeatPlot(u3, flowParams.L1, flowParams.L3, j2, filename); }
+    void PlotSpanwiseVelocity(std::string filename, int j2) {
+        if (gridParams.ThirdDimension()) HeatPlot(u2, flowParams.L1, flowParams.L3, j2, filename);
+    }
+    // -d/dz (u1 + U_) + d/dx u3
+    void PlotSpanwiseVorticity(std::string filename, int j2) {
+        NeumannNodal nn;
+        nodal(u1, nn);
+        nn += U_;
+        NeumannModal nm;
+        nn.ToModal(nm);
+        DirichletModal w = ddz(nm);
+        w *= -1;
+        w += ddx(u3);
+        HeatPlot(w, flowParams.L1, flowParams.L3, j2, filename);
+    }
+    // d/dz u1 - d/dx u3
+    void PlotPerturbationVorticity(std::string filename, int j2) {
+        DirichletModal w = ddz(u1);
+        w -= ddx(u3);
+        HeatPlot(w, flowParams.L1, flowParams.L3, j2, filename);
+    }
+    void PlotStreamwiseVelocity(std::string filename, int j2, bool includeBackground = true) {
+        if (includeBackground) {
+            NeumannNodal nn;
+            nodal(u1, nn);
+            nn += U_;
+            NeumannModal nm;
+            nn.ToModal(nm);
+            HeatPlot(nm, flowParams.L1, flowParams.L3, j2, filename);
+        } else {
+            HeatPlot(u1, flowParams.L1, flowParams.L3, j2, filename);
+        }
+    }
+    void PlotAll(std::string filename, bool includeBackground) {
+        const int j2 = gridParams.N2 / 2;
+        PlotPressure(imageDirectory + "/pressure/" + filename, j2);
+        PlotBuoyancy(imageDirectory + "/buoyancy/" + filename, j2, includeBackground);
+        PlotVerticalVelocity(imageDirectory + "/u3/" + filename, j2);
+        PlotSpanwiseVelocity(imageDirectory + "/u2/" + filename, j2);
+        PlotStreamwiseVelocity(imageDirectory + "/u1/" + filename, j2, includeBackground);
+        PlotPerturbationVorticity(imageDirectory + "/perturbvorticity/" + filename, j2);
+        if (includeBackground) PlotSpanwiseVorticity(imageDirectory + "/vorticity/" + filename, j2);
+    }
+    std::string imageDirectory;  // IMEXRK.h:612
+
     void flush_state() {
         u1.dev(false);
         u2.dev(false);
@@ -904,6 +974,7 @@ public:
     DirichletNodal U3;
 
 private:
+    static void nodal(const ModalField& m, NodalField& n) { m.ToNodal(n); }
     static sf_ctx* c() { return sfacade::ctx(); }
     static sf_field* state(int which) {
         sf_field* f = nullptr;
@@ -1031,9 +1102,24 @@ public:
     // nodal round trip
     void AddBackground() { Background(1); }
     void RemoveBackground() { Background(-1); }
-    // StateVector.h:476-500: the reference writes heat maps with matplotlib;
-    // here the directory is created and nothing is drawn
-    void PlotAll(std::string directory) const { MakeCleanDir(directory); }
+    // StateVector.h:476-500: heat maps of u1 (u2), u3, b and the spanwise
+    // vorticity -d/dz u1 + d/dx u3 at j2 = 0; TwoAndAHalf adds the vorticity
+    // of every spanwise slice
+    void PlotAll(std::string directory) const {
+        MakeCleanDir(directory);
+        const FlowParams& f = flowParams;
+        HeatPlot(u1, f.L1, f.L3, 0, directory + "/u1.png");
+        if (gridParams.ThirdDimension()) HeatPlot(u2, f.L1, f.L3, 0, directory + "/u2.png");
+        HeatPlot(u3, f.L1, f.L3, 0, directory + "/u3.png");
+        HeatPlot(b, f.L1, f.L3, 0, directory + "/b.png");
+        DirichletModal w = ddz(u1);
+        w *= -1;
+        w += ddx(u3);
+        HeatPlot(w, f.L1, f.L3, 0, directory + "/vorticity.png");
+        if (gridParams.dimensionality == Dimensionality::TwoAndAHalf)
+            for (int n = 0; n < gridParams.N2; n++)
+                HeatPlot(w, f.L1, f.L3, n, directory + "/vorticity" + std::to_string(n) + ".png");
+    }
 
     void Zero() {
         u1.Zero();
@@ -1100,10 +1186,10 @@ public:
     // StateVector.cpp:3-83: CFL-adaptive steps to time T that end exactly at T,
     // a progress line every 100 steps, .fields snapshots in snapshots/ (every
     // 100 steps and at the end) when snapshot is set, and the time integral of
-    // JoverK returned when calcmixing is set.  screenshot draws nothing here.
+    // JoverK returned when calcmixing is set, and when screenshot is set the
+    // solver's heat maps at the same steps in images-<run>/ (PlotAll).
     stratifloat FullEvolve(stratifloat T, StateVector& result, bool snapshot = false, bool screenshot = true,
                            bool calcmixing = false) const {
-        (void)screenshot;
         CopyToSolver();
         solver().SetBackground(InitialU);
         solver().FilterAll();
@@ -1111,13 +1197,16 @@ public:
         solver().RemoveDivergence(0.0f);
         stratifloat t = 0.0f;
         int step = 0;
-        solver().PrepareRun("", screenshot);
+        static int runnum = 0;
+        runnum++;
+        solver().PrepareRun(std::string("images-") + std::to_string(runnum) + "/", screenshot);
         MakeCleanDir("snapshots");
         const int stepinterval = 100;
         stratifloat mixing = 0;
         auto save = [&]() {
             solver().SaveFlow("snapshots/" + std::to_string(step) + "-" + std::to_string(t) + ".fields");
         };
+        auto shot = [&]() { solver().PlotAll(std::to_string(step) + "-" + std::to_string(t) + ".png", true); };
         while (t + 0.0001 < T) {
             if (step % stepinterval == 0) {
                 solver().CFL();
@@ -1129,13 +1218,17 @@ public:
                     solver().deltaT = remaining / remainingSteps;
                     solver().UpdateForTimestep();
                 }
+                if (screenshot) shot();
                 if (snapshot) save();
             }
             solver().TimeStep();
             if (calcmixing) mixing += solver().JoverK() * solver().deltaT;
             t += solver().deltaT;
             step++;
-            if (t + 0.0001 >= T && snapshot) save();
+            if (t + 0.0001 >= T) {
+                if (screenshot) shot();
+                if (snapshot) save();
+            }
         }
         CopyFromSolver(result);
         return mixing;
