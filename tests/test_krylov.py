@@ -78,6 +78,14 @@ def test_hookstep_least_squares(dense_out):
                 assert np.allclose(y, ls, rtol=1e-9, atol=1e-11)
             else:
                 assert np.linalg.norm(y) <= Delta * (1 + 1e-12)
+                # mu is the first point of the 1e-5 grid whose hookstep fits
+                # (NewtonKrylov.h:196-203): z(mu) = p d/(d^2 + mu) in the singular basis
+                U, sv, _ = np.linalg.svd(H, full_matrices=False)
+                p = U[0, :] * beta
+                z = lambda m: p * sv / (sv * sv + m)
+                assert np.linalg.norm(z(mur)) <= Delta * (1 + 1e-12)
+                assert np.linalg.norm(z(mur - 1e-5)) > Delta
+                assert abs(mur / 1e-5 - round(mur / 1e-5)) < 1e-3  # sums of 1e-5 in double
 
 
 def test_hessenberg_eigenvalues_match_lapack(dense_out):
