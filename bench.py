@@ -203,9 +203,23 @@ def newton_gmres(timeout=480):
         return {"error": "newton_bench could not run: %s" % e}
     if r.returncode != 0:
         return {"error": (r.stdout + r.stderr)[-400:]}
+    out = summarize_newton(r.stdout)
+    out["workload"] = ("Newton-GMRES to relative residual 1e-8 (at most 10 Newton steps), 2 held-out problems, "
+                       "256x1x256 2D, Re~U[1000,3000], Ri~U[0.08,0.2], F(x)=Phi_2(x)-x, FP64 stepping, K=60, "
+                       "GMRES tol 1e-2; learned = 4 x Conv1D(64 ch, k=5, dilations 1,2,4,8) BF16/MFMA network "
+                       "correcting the analytic advected-diffusion preconditioner (delta 0.3), trained on the "
+                       "GPU on the operator applications of 2 plain training runs")
+    return out
+
+
+def summarize_newton(text):
+    """Totals per method over the held-out problems of newton_bench's output
+    (its test/hist/loss lines): Newton steps, operator applications, seconds,
+    final residuals, fallbacks, the summed time to each residual decade (None
+    when some problem never reaches it) and how many problems reach it."""
     methods = {}
     loss = None
-    for line in r.stdout.splitlines():
+    for line in text.splitlines():
         t = line.split()
         if not t:
             continue
@@ -213,7 +227,7 @@ def newton_gmres(timeout=480):
             loss = {"first": float(t[1]), "last": float(t[2]), "samples": int(t[3]), "train_s": float(t[4])}
         elif t[0] == "test":
             m = methods.setdefault(t[4], {"newton_steps": 0, "matvecs": 0, "seconds": 0.0, "final_rel_residual": [],
-                                          "fallbacks": 0, "time_to": {}})
+                                          "fallbacks": 0, "time_to": {}, "problems_reaching": {}})
             m["newton_steps"] += int(t[5])
             m["matvecs"] += int(t[6])
             m["seconds"] += float(t[7])
@@ -225,6 +239,7 @@ def newton_gmres(timeout=480):
                 thr = 10.0 ** -e
                 hit = next((float(p.split("@")[1]) for p in t[3:] if float(p.split("@")[0]) <= thr), None)
                 key = "1e-%d" % e
+                m["problems_reaching"][key] = m["problems_reaching"].get(key, 0) + (hit is not None)
                 if hit is None:
                     m["time_to"][key] = None
                 elif m["time_to"].get(key, 0.0) is not None:
@@ -241,11 +256,6 @@ def newton_gmres(timeout=480):
                 ratio, level = p["time_to"][key] / l["time_to"][key], key
                 break
     return {
-        "workload": "Newton-GMRES to relative residual 1e-8 (at most 10 Newton steps), 2 held-out problems, "
-                    "256x1x256 2D, Re~U[1000,3000], Ri~U[0.08,0.2], F(x)=Phi_2(x)-x, FP64 stepping, K=60, "
-                    "GMRES tol 1e-2; learned = 4 x Conv1D(64 ch, k=5, dilations 1,2,4,8) BF16/MFMA network "
-                    "correcting the analytic advected-diffusion preconditioner (delta 0.3), trained on the "
-                    "GPU on the operator applications of 2 plain training runs",
         "methods": methods,
         "training_loss": loss,
         "time_ratio_plain_over_learned": ratio,
