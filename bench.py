@@ -279,7 +279,7 @@ def main():
                          "to scale FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: the IMEX-RK step is not sharded across GPUs in this build "
+        print("bench.py measures one GPU only: the IMEX-RK step runs on a single GPU and is not sharded "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -2,7 +2,7 @@
 //
 // The reference solves the GMRES least-squares problem with Eigen's JacobiSVD
 // (NewtonKrylov.h:191-215) and finds Arnoldi eigenvalues with Eigen's
-// EigenSolver (Arnoldi.h:57-60, 98-117).  Eigen is not part of this build, so
+// EigenSolver (Arnoldi.h:57-60, 98-117).  Eigen is not a dependency here, so
 // the same decompositions are written here from their textbook definitions:
 //   * jacobi_svd: one-sided (Hestenes) Jacobi SVD, A = U diag(d) V^T, thin U,
 //     singular values in decreasing order;
