@@ -465,9 +465,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (b % NXCD) * (nb / NXCD) + b / NXCD;
 }
 
-template <typename T, int M, bool FULL>
-__global__ __launch_bounds__((modal_mb<T, M>() * 64)) void k_modal(ModalArgs<T> A) {
-    constexpr int MB = modal_mb<T, M>();
+template <typename T, int M, bool FULL, int MB>
+__global__ __launch_bounds__(MB * 64) void k_modal(ModalArgs<T> A) {
     __shared__ cx<T> tile[64 * M * (MB + 1)];
     const int nt = gridDim.x;  // j1 tiles per ja
     const int t = xcd_tile(blockIdx.x + nt * blockIdx.y, nt * gridDim.y);
@@ -585,18 +584,38 @@ hipError_t launch_modal_batch(const ModalArgs<T>* args, const int* start, int nr
     return hipGetLastError();
 }
 
+template <typename T, int M, int MB>
+static void launch_modal_mb(const ModalArgs<T>& a, hipStream_t s) {
+    const dim3 grid((a.A1 + MB - 1) / MB, a.A2);
+    if (a.N3 == 64 * M) hipLaunchKernelGGL((k_modal<T, M, true, MB>), grid, dim3(MB * 64), 0, s, a);
+    else hipLaunchKernelGGL((k_modal<T, M, false, MB>), grid, dim3(MB * 64), 0, s, a);
+}
+
+// Grids with at most this many modes (one per CU) take one mode per block:
+// the GPU is far from full and latency sets the time, so each mode's wave
+// runs alone on its CU (2D 256 x 1 x 256 FP32: 82 against 112 us per step
+// with 8 modes per block); larger grids keep the wide tiles that make the
+// loads coalesce.
+inline int modal_small_grid() {
+    static const int n = []() {
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        return ncu;
+    }();
+    return n;
+}
+
 template <typename T>
 hipError_t launch_modal(const ModalArgs<T>& a, hipStream_t s) {
     const int M = (a.N3 + 63) / 64;
     switch (M) {
 #define SF_M(m)                                                                                              \
-    case m: {                                                                                                \
-        constexpr int MB = modal_mb<T, m>();                                                                 \
-        const dim3 grid((a.A1 + MB - 1) / MB, a.A2);                                                         \
-        if (a.N3 == 64 * m) hipLaunchKernelGGL((k_modal<T, m, true>), grid, dim3(MB * 64), 0, s, a);         \
-        else hipLaunchKernelGGL((k_modal<T, m, false>), grid, dim3(MB * 64), 0, s, a);                       \
-        break;                                                                                               \
-    }
+    case m:                                                                                                  \
+        if (a.A1 * a.A2 <= modal_small_grid())                                                               \
+            launch_modal_mb<T, m, 1>(a, s);                                                                  \
+        else                                                                                                 \
+            launch_modal_mb<T, m, modal_mb<T, m>()>(a, s);                                                   \
+        break;
         SF_M(1) SF_M(2) SF_M(3) SF_M(4) SF_M(5) SF_M(6) SF_M(7) SF_M(8)
 #undef SF_M
         default:
