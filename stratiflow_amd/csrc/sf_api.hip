@@ -1191,7 +1191,7 @@ struct Batch {
         auto best_x = [&](Group& g, bool mixed, int r1f) {
             double best = 1e30;
             int bz = 2, bgrid = 8 * ncu;
-            for (int zch : {2, 4, 8, 16}) {
+            for (int zch : {1, 2, 4, 8, 16}) {
                 for (int mult : {4, 8, 16, 0}) {
                     const int grid = mult ? mult * ncu : (1 << 30);
                     build_xgroup(g, zch, grid, r1f);

@@ -1037,9 +1037,10 @@ hipError_t launch_xpass(const XArgs<T>& a, int N1, int N2, bool threeD, hipStrea
 }
 
 // Levels per x-pass block.  Each block pays two extra ring levels, and the
-// last round of blocks may leave CUs idle; pick the height in [2, 48] that
+// last round of blocks may leave CUs idle; pick the height in [1, 48] that
 // minimises (rounds of resident blocks) x (levels per block + 2), given how
-// many blocks of this kernel the GPU keeps resident.
+// many blocks of this kernel the GPU keeps resident (small grids take one
+// level per block: more blocks in flight for the same single round).
 template <typename T, int N1, bool THREED, bool LIN>
 static int xpass_resident(int ncu) {
     constexpr int lds = xpass_lds<N1, THREED, LIN>() * int(sizeof(cx<T>));
@@ -1079,7 +1080,7 @@ int xpass_levels(int N1, int N2, int N3, bool threeD, int precision) {
     const int rows = threeD ? N2 : 1;
     long best = -1;
     int pick = 2;
-    for (int zc = 2; zc <= 48; ++zc) {
+    for (int zc = 1; zc <= 48; ++zc) {
         const long nblk = long((N3 + zc - 1) / zc) * rows;
         const long rounds = (nblk + slots - 1) / slots;
         const long cost = rounds * (zc + 2);
