@@ -15,9 +15,11 @@ def _report(tag, **kv):
 
 
 @pytest.mark.parametrize("prec,tol", [(8, 1e-11), (4, 2e-5)])
-@pytest.mark.parametrize("shape", [(48, 1, 64, 1), (24, 12, 32, 0)])
+@pytest.mark.parametrize("shape", [(48, 1, 64, 1), (24, 12, 32, 0), (96, 48, 64, 0), (64, 96, 48, 0)])
 def test_transforms_roundtrip(shape, prec, tol):
-    """ToNodal / ToModal against the oracle's FFT on the same spectra."""
+    """ToNodal / ToModal against the oracle's FFT on the same spectra (N2 = 48
+    and 96: the two-pass y transforms, the inverse loading its tiles straight
+    into LDS; N1 = 64: an odd A1 = 21)."""
     N1, N2, N3, dim = shape
     s, o = make_pair(N1, N2, N3, dim, gpu_prec=prec, project=False)
     import stratiflow_amd as sf
