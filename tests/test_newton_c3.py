@@ -16,6 +16,10 @@ context is compared with it:
     bound 0.1);
   * each of the leading FP32 eigenvalues lies within 0.06 max(1, |lambda|) of
     an FP64 one.
+The FP64 run is also held against the FP64 oracle's Newton step and
+eigenvalues from the same guess (tests/golden/c3_oracle_256x1x256.npz, written
+by tests/golden/make_c3_oracle.py): residuals, update norms and the ten
+leading eigenvalues.
 """
 import os
 import subprocess
@@ -51,6 +55,45 @@ def _fields(path, dtype):
     return np.fromfile(path, dtype=dtype).astype(np.float64).reshape(4, N1, N3)
 
 
+ORACLE_FIXTURE = os.path.join(ROOT, "tests", "golden", "c3_oracle_256x1x256.npz")
+
+
+def _against_oracle(out, r0, r1, ev):
+    """The FP64 run against the FP64 oracle's Newton step and eigenvalues
+    from the same guess (tests/golden/make_c3_oracle.py)."""
+    import oracle
+    from oracle import krylov
+    from stratiflow_amd import ic
+    d = np.load(ORACLE_FIXTURE)
+    o = oracle.Oracle(N1, 1, N3, oracle.TWO_D, 2 * np.pi / 0.44, 1.0, 10.0, 2000.0, 0.12, 1.0, precision=8)
+    z, zf, _, _ = o.grid()
+    o.set_background(ic.tanh_background(zf), ic.tanh_stratification(zf))
+
+    def modal(tag):
+        raw = np.fromfile(os.path.join(out, tag + ".fields"), dtype=np.float64).reshape(4, 1, N1, N3)
+        return krylov.enforce_bcs({"u1": o.to_modal(raw[0]), "u3": o.to_modal(raw[2]), "b": o.to_modal(raw[3])})
+
+    x0, x1 = modal("guess"), modal("solution")
+    g = np.array([np.linalg.norm(x0[n]) for n in ("u1", "u3", "b")])
+    assert np.allclose(g, d["guess_norms"], rtol=1e-9), "not the guess the fixture was made from"
+    dx = krylov.muladd(krylov.copy(x1), -1.0, x0)
+    dxn = np.array([np.linalg.norm(dx[n]) for n in ("u1", "u3", "b")])
+    cosg = krylov.dot(o, dx, x0, 0.12) / (krylov.norm(o, dx, 0.12) * krylov.norm(o, x0, 0.12))
+    print("vs FP64 oracle: residual before %.3e, after %.3e; update norms %s; update.guess %.3e" %
+          (abs(r0 - d["r0"]) / d["r0"], abs(r1 - d["r1"]) / d["r1"],
+           np.abs(dxn - d["dx_norms"]) / d["dx_norms"], abs(cosg - d["dx_dot_guess"])))
+    # the driver prints residuals to 7 digits; the finite-difference Jacobian
+    # (step 1e-7 |x|) and 60 Gram-Schmidt steps leave ~1e-7 between the two
+    assert abs(r0 - d["r0"]) / d["r0"] < 1e-6
+    assert abs(r1 - d["r1"]) / d["r1"] < 1e-5
+    assert np.all(np.abs(dxn - d["dx_norms"]) / d["dx_norms"] < 1e-5)
+    assert abs(cosg - d["dx_dot_guess"]) < 1e-5
+    for k, e in enumerate(d["ev"]):
+        dist = np.min(np.abs(ev - e))
+        print("eigenvalue %d oracle %.6f%+.6fi  nearest FP64 GPU distance %.3e" % (k, e.real, e.imag, dist))
+        assert dist < 1e-5 * max(1.0, abs(e))
+
+
 def test_newton_step_and_arnoldi_fp32_against_fp64(tmp_path):
     r32, s32, ev32 = _run("newton_c3", str(tmp_path / "f32"))
     r64, s64, ev64 = _run("newton_c3_f64", str(tmp_path / "f64"))
@@ -66,6 +109,8 @@ def test_newton_step_and_arnoldi_fp32_against_fp64(tmp_path):
             print("%s %s fp32 vs fp64 rel l2 %.3e" % (tag, n, e))
             assert e < bound, (tag, n, e)
     assert len(ev32) == 10 and len(ev64) == 10
+    if os.path.exists(ORACLE_FIXTURE):
+        _against_oracle(str(tmp_path / "f64"), r64, s64, ev64)
     for k, e in enumerate(ev32):
         d = np.min(np.abs(ev64 - e))
         print("eigenvalue %d fp32 %.6f%+.6fi  nearest fp64 distance %.3e" % (k, e.real, e.imag, d))
