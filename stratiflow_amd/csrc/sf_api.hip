@@ -191,8 +191,8 @@ struct Ctx {
         kx2 = dalloc<T>(A1);
         ky = dalloc<T>(A2);
         ky2 = dalloc<T>(A2);
-        tw1 = dalloc<V>(N1);
-        tw2 = dalloc<V>(N2);
+        tw1 = dalloc<V>(2 * N1);
+        tw2 = dalloc<V>(2 * N2);
         partial = dalloc<double>(2048);
         scal = dalloc<double>(16);
 
@@ -312,14 +312,14 @@ struct Ctx {
         SF_HIP(hipMemcpyAsync(ky2, hky2.data(), A2 * sizeof(T), hipMemcpyHostToDevice, stream));
         // FFT twiddles, computed in double
         auto tw = [&](int n, V* dst) {
-            std::vector<double> re(n), im(n);
+            std::vector<double> re(2 * n), im(2 * n);
             fft_twiddles(n, re.data(), im.data());
-            std::vector<V> w(n);
-            for (int k = 0; k < n; ++k) {
+            std::vector<V> w(2 * n);
+            for (int k = 0; k < 2 * n; ++k) {
                 w[k].x = T(re[k]);
                 w[k].y = T(im[k]);
             }
-            SF_HIP(hipMemcpyAsync(dst, w.data(), n * sizeof(V), hipMemcpyHostToDevice, stream));
+            SF_HIP(hipMemcpyAsync(dst, w.data(), 2 * n * sizeof(V), hipMemcpyHostToDevice, stream));
             SF_HIP(hipStreamSynchronize(stream));
         };
         tw(N1, tw1);
@@ -332,15 +332,15 @@ struct Ctx {
         if (r1f == 0 || fft_first_radix(N1, r1f) == fft_first_radix(N1, 0)) return tw1;
         SF_REQUIRE(r1f == 8, SF_ERR_ARG, "x FFT plans: first radix 0 (default) or 8");
         if (!tw1_r8) {
-            std::vector<double> re(N1), im(N1);
+            std::vector<double> re(2 * N1), im(2 * N1);
             fft_twiddles_plan(N1, 8, re.data(), im.data());
-            std::vector<V> w(N1);
-            for (int k = 0; k < N1; ++k) {
+            std::vector<V> w(2 * N1);
+            for (int k = 0; k < 2 * N1; ++k) {
                 w[k].x = T(re[k]);
                 w[k].y = T(im[k]);
             }
-            V* d = dalloc<V>(N1);
-            SF_HIP(hipMemcpyAsync(d, w.data(), N1 * sizeof(V), hipMemcpyHostToDevice, stream));
+            V* d = dalloc<V>(2 * N1);
+            SF_HIP(hipMemcpyAsync(d, w.data(), 2 * N1 * sizeof(V), hipMemcpyHostToDevice, stream));
             SF_HIP(hipStreamSynchronize(stream));
             tw1_r8 = d;
         }

@@ -13,7 +13,8 @@
 // 2/3/4/6/8-point kernels with compile-time twiddles; the pass-2 twiddles come
 // from a table tw[j*R2 + r] = exp(-2*pi*i*r*j/N) that the host fills in double
 // precision (fft_twiddle_table), so that each butterfly reads its twiddles at
-// compile-time offsets from one base.  SIGN = -1 is FFTW_FORWARD, +1 FFTW_BACKWARD; both unnormalised.
+// compile-time offsets from one base; entries N..2N-1 hold the same values in
+// the order of pass 2 on lane pairs, tw[N + j*R2 + (r%2)*R2/2 + r/2].  SIGN = -1 is FFTW_FORWARD, +1 FFTW_BACKWARD; both unnormalised.
 #pragma once
 #include "sf_common.h"
 
@@ -305,6 +306,83 @@ __device__ __forceinline__ void fft_pass2(int nl, LP lp, const cx<T>* __restrict
     sync_lds();
 }
 
+// Pass 2 on lane pairs: butterfly (line, j) of the radix-R2 pass is split over
+// two adjacent lanes (h = lane & 1).  Lane h takes the inputs r = 2m + h, applies
+// their twiddles and a DFT of R2/2 points; lane 1 multiplies by W^k, the pair
+// exchanges its halves through DPP (quad_perm [1,0,3,2]) and lane h writes the
+// outputs k + h*R2/2: X_k = E_k + W^k O_k, X_{k+R2/2} = E_k - W^k O_k.  Every
+// pair is whole: nl*R1*2 work items, even, lanes of a pair in one wave.  Each
+// lane reads its R2/2 twiddles as one contiguous run of the pair-ordered half
+// of the table.  The two lanes of a pair store R1*R2/2 values apart, in one
+// bank when that is a multiple of 32; pairing lanes l and l+32 instead
+// (v_permlane32_swap) avoids that conflict but costs a copy per exchanged value
+// and measured no faster.
+__device__ __forceinline__ float dpp_swap_pair(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double dpp_swap_pair(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp(int(b & 0xffffffff), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, int R1F = 0, typename LP>
+__device__ __forceinline__ void fft_pass2_pairs(int nl, LP lp, const cx<T>* __restrict__ tw, const int tid) {
+    using V = cx<T>;
+    constexpr int R1 = FftPlan<N, R1F>::R1, R2 = FftPlan<N, R1F>::R2, H = R2 / 2;
+    static_assert(R2 % 2 == 0, "pass 2 on lane pairs needs an even radix");
+    constexpr int PER = (NLMAX * R1 * 2 + BS - 1) / BS;
+    const int tot = nl * R1 * 2;
+    V v[PER][H];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int g = tid + i * BS;
+        if (g < tot) {
+            const int h = g & 1;
+            const int line = (g >> 1) / R1;
+            const int j = (g >> 1) - line * R1;
+            const V* Lp = lp(line) + (j + h * (R1 + 1)) * ES;
+            const V* twj = tw + N + j * R2 + h * H;
+#pragma unroll
+            for (int m = 0; m < H; ++m) {
+                V w = twj[m];
+                if (SIGN > 0) w.y = -w.y;
+                v[i][m] = cmul(Lp[2 * m * (R1 + 1) * ES], w);
+            }
+            reg_dft<T, H, SIGN>(v[i]);
+            // lane 1 of the pair: W^k O_k
+#pragma unroll
+            for (int k = 1; k < H; ++k) {
+                const V t = twv<T, SIGN, R2>(v[i][k], k);
+                v[i][k].x = h ? t.x : v[i][k].x;
+                v[i][k].y = h ? t.y : v[i][k].y;
+            }
+            const T sg = h ? T(-1) : T(1);
+#pragma unroll
+            for (int k = 0; k < H; ++k) {
+                const T px = dpp_swap_pair(v[i][k].x), py = dpp_swap_pair(v[i][k].y);
+                v[i][k].x = px + sg * v[i][k].x;
+                v[i][k].y = py + sg * v[i][k].y;
+            }
+        }
+    }
+    sync_lds();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int g = tid + i * BS;
+        if (g < tot) {
+            const int h = g & 1;
+            const int line = (g >> 1) / R1;
+            const int j = (g >> 1) - line * R1;
+            V* L = lp(line) + (j + R1 * H * h) * ES;
+#pragma unroll
+            for (int k = 0; k < H; ++k) L[R1 * k * ES] = v[i][k];
+        }
+    }
+    sync_lds();
+}
+
 template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, typename LP>
 __device__ __forceinline__ void fft_single(int nl, LP lp, const int tid) {
     using V = cx<T>;
@@ -339,12 +417,19 @@ __device__ __forceinline__ void fft_single(int nl, LP lp, const int tid) {
 // filling the lines; on return the results are visible to all threads.  tid
 // is the thread's index in [0, BS) for the distribution of the work items (a
 // block may permute it to spread partially filled rounds over its waves).
-template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, int R1F = 0, typename LP>
+// PAIRS (contiguous lines, even R2) runs pass 2 on lane pairs; it needs tid
+// and tid ^ 1 on adjacent lanes of one wave, which a rotation of tid by whole
+// waves keeps, and all BS threads calling.
+template <typename T, int N, int NLMAX, int BS, int SIGN, int ES = 1, bool IL = false, int R1F = 0,
+          bool PAIRS = false, typename LP>
 __device__ __forceinline__ void lds_fft_lines(int nl, LP lp, const cx<T>* __restrict__ tw,
                                               const int tid = int(threadIdx.x)) {
     if constexpr (FftPlan<N>::two) {
         fft_pass1<T, N, NLMAX, BS, SIGN, ES, IL, R1F>(nl, lp, tid);
-        fft_pass2<T, N, NLMAX, BS, SIGN, ES, IL, R1F>(nl, lp, tw, tid);
+        if constexpr (PAIRS && !IL && FftPlan<N, R1F>::R2 % 2 == 0)
+            fft_pass2_pairs<T, N, NLMAX, BS, SIGN, ES, R1F>(nl, lp, tw, tid);
+        else
+            fft_pass2<T, N, NLMAX, BS, SIGN, ES, IL, R1F>(nl, lp, tw, tid);
     } else {
         fft_single<T, N, NLMAX, BS, SIGN, ES>(nl, lp, tid);
         (void)tw;
@@ -358,8 +443,8 @@ __device__ __forceinline__ void lds_fft(cx<T>* buf, const cx<T>* __restrict__ tw
     lds_fft_lines<T, N, NL, BS, SIGN>(NL, [buf](int line) { return buf + line * LDP; }, tw, tid);
 }
 
-// host: twiddle table of length N in the layout fft_pass2 reads (entry count
-// N) for the plan FftPlan<N, R1F>
+// host: twiddle table of 2N entries for the plan FftPlan<N, R1F>: the layout
+// fft_pass2 reads, then the same values in the order fft_pass2_pairs reads
 template <int N, int R1F = 0>
 void fft_twiddle_table(double* re, double* im) {
     constexpr int R1 = FftPlan<N, R1F>::R1, R2 = FftPlan<N, R1F>::R2;
@@ -373,6 +458,17 @@ void fft_twiddle_table(double* re, double* im) {
                 const double a = -2.0 * cpi * double((r * j) % N) / double(N);
                 re[j * R2 + r] = std::cos(a);
                 im[j * R2 + r] = std::sin(a);
+            }
+    }
+    for (int k = 0; k < N; ++k) {
+        re[N + k] = re[k];
+        im[N + k] = im[k];
+    }
+    if constexpr (FftPlan<N, R1F>::two && R2 % 2 == 0) {
+        for (int j = 0; j < R1; ++j)
+            for (int r = 0; r < R2; ++r) {
+                re[N + j * R2 + (r & 1) * (R2 / 2) + r / 2] = re[j * R2 + r];
+                im[N + j * R2 + (r & 1) * (R2 / 2) + r / 2] = im[j * R2 + r];
             }
     }
 }

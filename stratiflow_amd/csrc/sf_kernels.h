@@ -178,7 +178,8 @@ bool tz_supported(int n);
 // z levels per x-pass block for this grid (the block height that minimises
 // rounds of resident blocks x (levels + 2))
 int xpass_levels(int N1, int N2, int N3, bool threeD, int precision);
-// twiddle table of an n-point LDS FFT (the layout its second pass reads)
+// twiddle table of an n-point LDS FFT: 2n entries (the layout its second pass
+// reads, then the same values in the order of the second pass on lane pairs)
 void fft_twiddles(int n, double* re, double* im);
 // the same for the plan with first radix r1f (8 or 16 where n allows it,
 // else the default), and that plan's first radix

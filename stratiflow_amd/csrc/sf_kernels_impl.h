@@ -770,7 +770,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
     // the previous level's store, consumed by this level's store
     if (zbeg + 1 < zend) load(zbeg + 2);
     sync_lds();
-    lds_fft_lines<T, N1, 3 * NI, XB, -1, 1, false, R1F>(3 * NI, [ring](int line) { return ring + line * LDP; }, A.tw,
+    lds_fft_lines<T, N1, 3 * NI, XB, -1, 1, false, R1F, true>(3 * NI, [ring](int line) { return ring + line * LDP; }, A.tw,
                                                         tid);
 
     for (int z = zbeg; z < zend; ++z) {
@@ -876,7 +876,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
         sync_lds();
         {
             const int s2 = slot(z + 2) * NI;
-            lds_fft_lines<T, N1, NB, XB, -1, 1, false, R1F>(more ? NB : NP,
+            lds_fft_lines<T, N1, NB, XB, -1, 1, false, R1F, true>(more ? NB : NP,
                                              [prd, s2](int line) {
                                                  return prd + (line + (line >= NP ? s2 : 0)) * LDP;
                                              },

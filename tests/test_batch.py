@@ -6,7 +6,10 @@ per-mode code, only the assignment of work to blocks differs.  The exception
 is an x-pass FFT plan other than the runs' own (first radix 8): it rounds
 differently, as FFTW's timed plans do (FFTW_PATIENT, Field.h:730-991), so those
 runs agree with their twins to rounding (FP32 1e-5, FP64 1e-12 of the field's
-largest value after a few steps, ten times that for the pressure).
+largest value after a few steps, ten times that for the pressure, or in FP32
+three times the distance of the FP32 twin from an FP64 twin, whichever is
+larger: the pressure of a small run is fixed only to a few 1e-4 of its maximum
+by FP32 rounding).
 """
 import numpy as np
 import pytest
@@ -18,13 +21,16 @@ RUNS = [(32, 48, 500.0, 0.05, 0.01), (48, 64, 1200.0, 0.12, 0.01), (32, 96, 2000
         (64, 128, 4000.0, 0.25, 0.01), (48, 160, 800.0, 0.1, 0.008), (96, 64, 3000.0, 0.15, 0.01)]
 
 
-def same(a, c, exact, prec, field="u1"):
+def same(a, c, exact, prec, field="u1", c64=None):
     """Bit-identical, or within rounding of the other FFT plan (ten times
-    looser for the pressure, which the projection rescales by 1/h)."""
+    looser for the pressure, which the projection rescales by 1/h; in FP32 at
+    least three times the twin's own distance from its FP64 twin c64)."""
     if exact:
         return np.array_equal(a, c)
-    tol = (1e-5 if prec == 4 else 1e-12) * (10 if field == "p" else 1)
-    return np.abs(a - c).max() <= tol * np.abs(c).max()
+    tol = (1e-5 if prec == 4 else 1e-12) * (10 if field == "p" else 1) * np.abs(c).max()
+    if c64 is not None:
+        tol = max(tol, 3 * np.abs(c - c64).max())
+    return np.abs(a - c).max() <= tol
 
 
 def _make(N1, N3, Re, Ri, dt, seed, prec):
@@ -53,25 +59,30 @@ def test_batch_matches_individual_runs(prec):
     print("batch", info, flush=True)
     assert info["xpass_groups"] == 4 and info["modal_groups"] == 3
     exact = info["xpass_plan"] != "mixed_r8"
+    # FP64 twins measure the FP32 rounding that another FFT plan may change
+    twins64 = [_make(*r, seed=100 + i, prec=8) for i, r in enumerate(RUNS)] if (prec == 4 and not exact) else []
     b.step(5)
-    for t in twins:
+    for t in twins + twins64:
         t.TimeStep(5)
     b.step(3)  # graph replays only
-    for t in twins:
+    for t in twins + twins64:
         t.TimeStep(3)
     for i, (s, t) in enumerate(zip(batched, twins)):
         for f in ("u1", "u3", "b", "p"):
             a, c = s.get(f), t.get(f)
+            c64 = twins64[i].get(f) if twins64 else None
             assert np.isfinite(a).all()
-            assert same(a, c, exact, prec, f), (i, f, np.abs(a - c).max() / np.abs(c).max())
+            assert same(a, c, exact, prec, f, c64), (i, f, np.abs(a - c).max() / np.abs(c).max())
     # a dt change is picked up (the graph is rebuilt)
     batched[0].deltaT = 0.004
     twins[0].deltaT = 0.004
+    if twins64:
+        twins64[0].deltaT = 0.004
     b.step(2)
-    for t in twins:
+    for t in twins + twins64:
         t.TimeStep(2)
-    for s, t in zip(batched, twins):
-        assert same(s.get("u1"), t.get("u1"), exact, prec)
+    for i, (s, t) in enumerate(zip(batched, twins)):
+        assert same(s.get("u1"), t.get("u1"), exact, prec, "u1", twins64[i].get("u1") if twins64 else None)
     b.close()
 
 
@@ -83,6 +94,7 @@ def test_every_batch_plan_matches_individual_runs(prec):
     import stratiflow_amd as sf
     batched = [_make(*r, seed=200 + i, prec=prec) for i, r in enumerate(RUNS)]
     twins = [_make(*r, seed=200 + i, prec=prec) for i, r in enumerate(RUNS)]
+    twins64 = [_make(*r, seed=200 + i, prec=8) for i, r in enumerate(RUNS)] if prec == 4 else []
     b = sf.Batch(batched)
     info = b.info()
     print("tuned", info, flush=True)
@@ -101,11 +113,12 @@ def test_every_batch_plan_matches_individual_runs(prec):
         assert info["launches_per_step"] == 3 * ((1 if x else 4) + (1 if m else 3)), info
         exact = exact and x != sf.BATCH_MIXED_R8
         b.step(2)
-        for tw in twins:
+        for tw in twins + twins64:
             tw.TimeStep(2)
         for i, (s, tw) in enumerate(zip(batched, twins)):
             for f in ("u1", "u3", "b", "p"):
                 a, c = s.get(f), tw.get(f)
+                c64 = twins64[i].get(f) if twins64 else None
                 assert np.isfinite(a).all()
-                assert same(a, c, exact, prec, f), (x, m, i, f, np.abs(a - c).max() / np.abs(c).max())
+                assert same(a, c, exact, prec, f, c64), (x, m, i, f, np.abs(a - c).max() / np.abs(c).max())
     b.close()
