@@ -122,3 +122,26 @@ def test_every_batch_plan_matches_individual_runs(prec):
                 assert np.isfinite(a).all()
                 assert same(a, c, exact, prec, f, c64), (x, m, i, f, np.abs(a - c).max() / np.abs(c).max())
     b.close()
+
+
+def test_radix8_plans_of_larger_grids_fp64():
+    """The first-radix-8 x FFT plans of N1 = 128, 192 and 256 (8 x 16, 8 x 24,
+    8 x 32: lane pairs of radix 8, 12 and 16) against the runs' own default
+    plans (16 x 8, 16 x 12, 16 x 16), in FP64 after four steps."""
+    import stratiflow_amd as sf
+    runs = [(128, 64, 1500.0, 0.1, 0.01), (192, 96, 2500.0, 0.15, 0.01), (256, 48, 800.0, 0.2, 0.008)]
+    batched = [_make(*r, seed=300 + i, prec=8) for i, r in enumerate(runs)]
+    twins = [_make(*r, seed=300 + i, prec=8) for i, r in enumerate(runs)]
+    b = sf.Batch(batched)
+    b.select(sf.BATCH_MIXED_R8, sf.BATCH_MIXED)
+    assert b.info()["xpass_plan"] == "mixed_r8"
+    b.step(4)
+    for t in twins:
+        t.TimeStep(4)
+    for i, (s, t) in enumerate(zip(batched, twins)):
+        for f in ("u1", "u3", "b", "p"):
+            a, c = s.get(f), t.get(f)
+            err = np.abs(a - c).max() / np.abs(c).max()
+            print(runs[i][:2], f, "%.2e" % err, flush=True)
+            assert err <= (1e-11 if f == "p" else 1e-12), (i, f, err)
+    b.close()

@@ -1,7 +1,9 @@
 """GPU path vs oracle at the edges of the supported grids: vertical columns
 that do not fill whole wavefronts (N3 below 64 or not a multiple of 64), the
-smallest 3D grid (N2 = 6), the 256x1x256 Newton-search grid, and the largest
-sizes (N1 = N3 = 512).  FP64 contexts support N1 <= 384, so the largest grid
+smallest 3D grid (N2 = 6), the 256x1x256 Newton-search grid, the largest
+sizes (N1 = N3 = 512), and N1 = 192, whose x FFT (16 x 12) runs its second
+pass on lane pairs of radix 6 (the other N1 cover radix 2, 3, 4, 8, 12 and 16
+per lane, and N1 = 48 the unsplit radix-3 pass).  FP64 contexts support N1 <= 384, so the largest grid
 runs in FP32 against the FP32 and FP64 oracles."""
 import numpy as np
 import pytest
@@ -12,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("shape", [(256, 1, 256, 1), (64, 1, 100, 1), (48, 1, 512, 1), (12, 6, 16, 0),
-                                   (96, 48, 36, 0), (96, 48, 100, 0)])
+                                   (96, 48, 36, 0), (96, 48, 100, 0), (192, 1, 64, 1), (192, 12, 36, 0)])
 def test_edge_grids_fp64(shape):
     """Three full steps in FP64: every prognostic field to 1e-9 (relative L2)."""
     N1, N2, N3, dim = shape
