@@ -107,6 +107,9 @@ struct Ctx {
     V* Y[4] = {nullptr};
     V* Yb[4] = {nullptr};
     V* X[8] = {nullptr};
+    // X[] (the 3D x-pass outputs) in tiles of 16 modes where the paired
+    // forward y transform reads them as contiguous runs (yfwd_tiles)
+    bool ytiles = false;
     // Spectra the nodal fields were last populated from.  The reference's
     // products read the nodal fields U1..B, which PopulateNodalVariables
     // refreshes at the end of every stage but which RemoveDivergence does not
@@ -225,6 +228,7 @@ struct Ctx {
             const size_t ny = size_t(N2) * A1 * N3;
             for (int i = 0; i < 4; ++i) Y[i] = dalloc<V>(ny);
             for (int i = 0; i < 8; ++i) X[i] = dalloc<V>(ny);
+            ytiles = yfwd_tiles(N2, A1, int(sizeof(T)));
         }
         set_flow(f);
         std::vector<double> U(N3, 0.0);
@@ -420,6 +424,8 @@ struct Ctx {
             x.tw = tw1;
             x.scale = T(1);
             x.linear = linear ? 1 : 0;
+            x.N2 = N2;
+            x.tiled = ytiles ? 1 : 0;
             for (int i = 0; i < 4; ++i) x.base[i] = linear ? Yb[i] : nullptr;
             check(launch_xpass<T>(x, N1, N2, true, stream), "xpass");
             mark(SF_K_XPASS);
@@ -429,6 +435,7 @@ struct Ctx {
             f.A2 = A2;
             f.nf = 4;
             f.paired = 1;
+            f.tiled = ytiles ? 1 : 0;
             for (int r = 0; r < 4; ++r) {
                 f.in[2 * r] = X[r];
                 f.in[2 * r + 1] = X[4 + r];

@@ -35,6 +35,8 @@ __host__ __device__ inline size_t sidx(int row, int z, int j1, int N3, int A1) {
 template <typename T>
 struct XArgs {
     int N1, N3, N3p, A1, zch;            // zch: z levels per block
+    int N2;                              // 3D: y points (rows of the outputs)
+    int tiled;                           // 3D: outputs in tiles of 16 modes (ytile_idx)
     const cx<T>* in[4];                  // u1, u2, u3, b spectra
     cx<T>* out[8];                       // 3D: A1,A2,A3,AB,B1,B2,B3,BB ; 2D: Nl1,-,Nl3,NlB
     const T* zc;
@@ -55,7 +57,17 @@ struct YArgs {
     const cx<T>* tw;
     T scale;
     int paired;       // forward: out[r] = scale*(F(in[2r]) + i ky F(in[2r+1]))
+    int tiled;        // paired forward: inputs in tiles of 16 modes (ytile_idx)
 };
+
+// Offset of (n2, z, j1) in an x-spectral field stored in tiles of 16 modes,
+// [z][j1/16][n2][j1%16] (A1 a multiple of 16): what the 3D x-pass writes and
+// the paired forward y transform reads when the grid allows it
+// (yfwd_tiles), so that a y-transform block's 16-mode tile is one
+// contiguous run of N2*16 values.
+__host__ __device__ inline size_t ytile_idx(int n2, int z, int j1, int N2, int A1) {
+    return ((size_t(z) * size_t(A1 >> 4) + size_t(j1 >> 4)) * size_t(N2) + size_t(n2)) * 16 + size_t(j1 & 15);
+}
 
 // Single-field x transforms between nodal and x-spectral layouts.
 template <typename T>
@@ -184,6 +196,9 @@ void fft_twiddles(int n, double* re, double* im);
 // the same for the plan with first radix r1f (8 or 16 where n allows it,
 // else the default), and that plan's first radix
 void fft_twiddles_plan(int n, int r1f, double* re, double* im);
+// the paired forward y transform of this grid runs on tiles of 16 modes read
+// as contiguous runs (A1 a multiple of 16 and k_yfwd_seq's LDS tile fits)
+bool yfwd_tiles(int N2, int A1, int precision);
 int fft_first_radix(int n, int r1f);
 int max_rows_per_lane();
 // blocks (and so partial sums) of the two-pass FP64 reductions (dot, max-abs)

@@ -897,7 +897,7 @@ __device__ __forceinline__ void xpass_body(const XArgs<T>& A, const int bx, cons
             }
             const T kxv = A.kx[k];
             const T ms = -A.scale;
-            const size_t off = sidx(n2, z, k, N3, A1);
+            const size_t off = (THREED && A.tiled) ? ytile_idx(n2, z, k, A.N2, A1) : sidx(n2, z, k, N3, A1);
             // r1 -= ikx P1 + P2 ; r3 -= ikx P3 + ddz P4 ; rB -= P5 + ikx P6
             A.out[0][off] = cscale(ms, cadd(cik(kxv, Fa[0]), Fb[0]));
             A.out[2][off] = cscale(ms, cadd(cik(kxv, Fa[1]), Fb[1]));
@@ -1321,7 +1321,9 @@ struct YDma {
 };
 
 // Paired forward y transform: F(A) is kept in registers at the active rows
-// while B is loaded into the same LDS.
+// while B is loaded into the same LDS.  The inputs are in the sidx layout, or
+// in tiles of 16 modes (A.tiled, ytile_idx) where a block's tile of each field
+// is one contiguous run.
 template <typename T, int N2, int YJ>
 __global__ __launch_bounds__(BS) void k_yfwd_seq(YArgs<T> A) {
     using V = cx<T>;
@@ -1345,7 +1347,8 @@ __global__ __launch_bounds__(BS) void k_yfwd_seq(YArgs<T> A) {
             const rsrc_t r = make_rsrc(A.in[f0 + q], fbytes);
             for (int i = wave; i < N2 / D::RPI; i += BS / 64) {
                 const int row = i * D::RPI + lrow;
-                const int off = jok ? int(sidx(row, z, j10 + col, N3, A1) * sizeof(V)) : BUF_OOB;
+                const size_t e = A.tiled ? ytile_idx(row, z, j10 + col, N2, A1) : sidx(row, z, j10 + col, N3, A1);
+                const int off = jok ? int(e * sizeof(V)) : BUF_OOB;
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
                     r, (__attribute__((address_space(3))) void*)(buf + i * D::RPI * YJ), 16, off, 0, 0, 0);
             }
@@ -1441,6 +1444,19 @@ static hipError_t launch_y_v(const YArgs<T>& a, bool inv, hipStream_t s) {
     }
 }
 
+// the paired forward transform of nin fields runs k_yfwd_seq (16-mode tiles
+// straight into LDS)
+template <typename T, int N2>
+static bool yfwd_seq_fits(int A1, int nin) {
+    if constexpr (FftPlan<N2>::two && YDma<T, N2, 16>::LDS <= 160 * 1024) {
+        return YDma<T, N2, 16>::fits(A1, nin);
+    } else {
+        (void)A1;
+        (void)nin;
+        return false;
+    }
+}
+
 template <typename T, int N2>
 static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
     // the inverse loads 32-wide tiles (256-byte runs) straight into LDS, the
@@ -1461,16 +1477,16 @@ static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
             return hipGetLastError();
         }
     }
-    if (inv || !a.paired) return launch_y_v<T, N2, 16, 1>(a, inv, s);
-    if constexpr (FftPlan<N2>::two && YDma<T, N2, 16>::LDS <= 160 * 1024) {
-        using D = YDma<T, N2, 16>;
-        if (D::fits(a.A1, 2 * a.nf)) {
+    if (inv || !a.paired) return a.tiled ? hipErrorInvalidValue : launch_y_v<T, N2, 16, 1>(a, inv, s);
+    if (yfwd_seq_fits<T, N2>(a.A1, 2 * a.nf)) {
+        if constexpr (FftPlan<N2>::two && YDma<T, N2, 16>::LDS <= 160 * 1024) {
             dim3 grid(a.N3, (a.A1 + 15) / 16);
             hipLaunchKernelGGL((k_yfwd_seq<T, N2, 16>), grid, dim3(BS), 0, s, a);
             return hipGetLastError();
         }
     }
-    return launch_y_v<T, N2, 8, 2>(a, inv, s);
+    // the register path reads the untiled layout only
+    return a.tiled ? hipErrorInvalidValue : launch_y_v<T, N2, 8, 2>(a, inv, s);
 }
 
 template <typename T>
@@ -1497,6 +1513,21 @@ hipError_t launch_yfwd(const YArgs<T>& a, int N2, hipStream_t s) {
             return hipErrorInvalidValue;
     }
 }
+
+#ifdef SF_KERNELS_HOST_DEFS
+bool yfwd_tiles(int N2, int A1, int precision) {
+    if (A1 % 16 != 0) return false;
+    switch (N2) {
+#define SF_CASE(n) \
+    case n:        \
+        return precision == 8 ? yfwd_seq_fits<double, n>(A1, 8) : yfwd_seq_fits<float, n>(A1, 8);
+        SF_SIZES(SF_CASE)
+#undef SF_CASE
+        default:
+            return false;
+    }
+}
+#endif
 
 // ---------------------------------------------------------------------------
 // single-field x transforms (ToNodal / ToModal for API fields), one block per
