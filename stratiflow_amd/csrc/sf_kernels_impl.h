@@ -1459,6 +1459,9 @@ static bool yfwd_seq_fits(int A1, int nin) {
 
 template <typename T, int N2>
 static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
+    // tiled inputs exist only for the paired forward transform on k_yfwd_seq
+    if (a.tiled && (inv || !a.paired || !yfwd_seq_fits<T, N2>(a.A1, 2 * a.nf) || a.A1 % 16 != 0))
+        return hipErrorInvalidValue;
     // the inverse loads 32-wide tiles (256-byte runs) straight into LDS, the
     // paired forward pass 16-wide tiles of one field at a time (LDS for six
     // blocks per CU in FP32; 32-wide tiles, or both fields of a pair in LDS,
@@ -1477,7 +1480,7 @@ static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
             return hipGetLastError();
         }
     }
-    if (inv || !a.paired) return a.tiled ? hipErrorInvalidValue : launch_y_v<T, N2, 16, 1>(a, inv, s);
+    if (inv || !a.paired) return launch_y_v<T, N2, 16, 1>(a, inv, s);
     if (yfwd_seq_fits<T, N2>(a.A1, 2 * a.nf)) {
         if constexpr (FftPlan<N2>::two && YDma<T, N2, 16>::LDS <= 160 * 1024) {
             dim3 grid(a.N3, (a.A1 + 15) / 16);
@@ -1485,8 +1488,7 @@ static hipError_t launch_y_n(const YArgs<T>& a, bool inv, hipStream_t s) {
             return hipGetLastError();
         }
     }
-    // the register path reads the untiled layout only
-    return a.tiled ? hipErrorInvalidValue : launch_y_v<T, N2, 8, 2>(a, inv, s);
+    return launch_y_v<T, N2, 8, 2>(a, inv, s);
 }
 
 template <typename T>
