@@ -332,6 +332,7 @@ __device__ __forceinline__ void fft_pass2_pairs(int nl, LP lp, const cx<T>* __re
     using V = cx<T>;
     constexpr int R1 = FftPlan<N, R1F>::R1, R2 = FftPlan<N, R1F>::R2, H = R2 / 2;
     static_assert(R2 % 2 == 0, "pass 2 on lane pairs needs an even radix");
+    static_assert(BS % 64 == 0, "pass 2 on lane pairs needs whole waves");
     constexpr int PER = (NLMAX * R1 * 2 + BS - 1) / BS;
     const int tot = nl * R1 * 2;
     V v[PER][H];
